@@ -1,0 +1,154 @@
+"""Writes tests/golden/ref_vectors.json from the reference's own compiled CPU classes
+and drivers (oracle/_ref/libref.so, built by oracle/build_ref.sh from the reference
+sources).  Run in the container that has the reference sources:
+
+    python tests/golden/make_golden.py
+
+The fixtures are data only: inputs, seeds and the reference's outputs.  Spin arrays
+are stored whole (hex words) for small lattices and as SHA-256 of their
+little-endian bytes otherwise.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+import oracle as O  # noqa: E402
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.json")
+
+
+def hexw(a) -> list[str]:
+    return [f"{int(v):016x}" for v in np.asarray(a, dtype=np.uint64)]
+
+
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<u8").tobytes()).hexdigest()
+
+
+def f32bits(a) -> list[str]:
+    return [f"{int(v):08x}" for v in np.asarray(a, dtype=np.float32).view(np.uint32)]
+
+
+def f64bits(a) -> list[str]:
+    return [f"{int(v):016x}" for v in np.asarray(a, dtype=np.float64).view(np.uint64)]
+
+
+def engines():
+    R = O.ref()
+    out = {}
+    for seed in (5489, 1234, 42):
+        a = np.zeros(10000, dtype=np.uint64)
+        R.ref_mt64(seed, a, a.size)
+        b = np.zeros(10000, dtype=np.uint32)
+        R.ref_mt32(seed, b, b.size)
+        u = np.zeros(16, dtype=np.float64)
+        R.ref_uniform_real64(seed, u, u.size)
+        out[str(seed)] = dict(mt64_first=hexw(a[:16]), mt64_10000=f"{int(a[-1]):016x}",
+                              mt32_first=[int(x) for x in b[:16]], mt32_10000=int(b[-1]),
+                              uniform_real64_first=f64bits(u))
+    pts = {}
+    for nT in (2, 3, 8, 32):
+        p = np.zeros(200, dtype=np.int32)
+        us = np.zeros(200, dtype=np.float64)
+        R.ref_pt_draws(nT, p, us, p.size)
+        pts[str(nT)] = dict(pairs=[int(x) for x in p], us=f64bits(us))
+    return dict(engines=out, pt_draws=pts)
+
+
+def tables():
+    out = []
+    for kind, T, h in [(2, 1.0, 0.0), (2, 0.5, 0.0), (2, 2.0, 0.0), (2, 0.7, 0.37), (2, 0.01, 0.0),
+                       (1, 3.0, 0.0), (1, 0.1, 0.0), (1, 1.3, 0.25), (0, 1.3, 0.0), (0, 0.4, -0.2)]:
+        r = O.RefSystem(kind, 4, T, h)
+        out.append(dict(kind=kind, T=T, h=h, boltz=f64bits(r.state()["boltz"])))
+    return out
+
+
+def sweeps():
+    cases = []
+    for kind, L, T, h, seed, nsw in [
+        (2, 8, 1.0, 0.0, 1234, 10), (2, 6, 0.8, 0.0, 7, 12), (2, 5, 1.5, 0.0, 99, 8), (2, 7, 1.1, 0.3, 5, 6),
+        (2, 16, 1.0, 0.0, 1234, 4), (2, 4, 2.0, 0.0, 3, 20),
+        (1, 16, 1.3, 0.0, 1234, 12), (1, 9, 0.9, 0.0, 11, 15), (1, 64, 2.0, 0.0, 1234, 3),
+        (0, 16, 1.3, 0.0, 1234, 12), (0, 11, 0.6, 0.2, 17, 9), (0, 8, 1.0, 0.0, 2, 20),
+    ]:
+        r = O.RefSystem(kind, L, T, h)
+        r.seed_like_main(seed)
+        st0 = r.state()
+        E0 = r.measure()
+        recs = []
+        for k in range(nsw):
+            r.sweep(1)
+            recs.append(f32bits(r.measure()))
+        st = r.state()
+        c = dict(kind=kind, L=L, T=T, h=h, seed=seed, sweeps=nsw, E0=f32bits(E0), E=recs,
+                 s0_sha=sha(st0["s"]), s_sha=sha(st["s"]), jx_sha=sha(st0["Jx"][0::2]),
+                 jy_sha=sha(st0["Jy"][0::2]))
+        if kind == 2:
+            c["jz_sha"] = sha(st0["Jz"][0::2])
+        if st["s"].size <= 512:
+            c["s"] = hexw(st["s"])
+        cases.append(c)
+    return cases
+
+
+def temperature_schedule():
+    """2D periodic L=16, per-sweep set_T on the linear schedule 3.0 -> 0.1, measure
+    every 10 sweeps (the reference classes driven directly)."""
+    kind, L, seed, n, stride = 1, 16, 1234, 200, 10
+    Ts = [3.0 - 2.9 * k / (n - 1) for k in range(n)]
+    r = O.RefSystem(kind, L, Ts[0])
+    r.seed_like_main(seed)
+    recs = []
+    for k in range(n):
+        r.set_T(Ts[k])
+        r.sweep(1)
+        if k % stride == 0:
+            recs.append(f32bits(r.measure()))
+    return dict(kind=kind, L=L, seed=seed, T=Ts, stride=stride, E=recs, s_sha=sha(r.state()["s"]))
+
+
+def pt_runs():
+    out = []
+    for kind, L, nT, sweeps, nsteps, h in [(2, 6, 4, 3, 40, 0.0), (2, 8, 8, 10, 20, 0.0), (1, 16, 4, 5, 40, 0.0),
+                                             (0, 12, 6, 2, 40, 0.0), (2, 6, 5, 2, 30, 0.3)]:
+        T = [0.5 * 4 ** (k / (nT - 1)) for k in range(nT)][::-1]
+        r = O.ref_pt_run(kind, L, np.array(T), h, 1234, sweeps, nsteps)
+        out.append(dict(kind=kind, L=L, T=T, h=h, seed=1234, sweeps=sweeps, nsteps=nsteps,
+                        pairs=[int(x) for x in r["pairs"]], us=f64bits(r["us"]), masks=hexw(r["masks"]),
+                        accept=f64bits(r["accept"]), E=[f32bits(e) for e in r["E"]],
+                        spins_sha=[sha(s) for s in r["spins"]]))
+    return out
+
+
+def anneal_geometric():
+    out = []
+    for kind, L, T0, alfa, sw, ns, stride in [(1, 16, 3.0, 0.9, 4, 10, 2), (2, 6, 2.5, 0.8, 3, 8, 1),
+                                               (0, 12, 2.0, 0.95, 2, 6, 1)]:
+        txt = O.ref_anneal_geometric(kind, L, T0, alfa, sw, ns, stride, 0.0, 1234)
+        out.append(dict(kind=kind, L=L, T0=T0, alfa=alfa, sweeps=sw, nsteps=ns, stride=stride, seed=1234,
+                        text=txt))
+    return out
+
+
+def main():
+    if not O.ref_available():
+        O.build_ref()
+    doc = dict(
+        about="Outputs of the reference's compiled CPU classes/drivers (see make_golden.py).",
+        **engines(), tables=tables(), sweeps=sweeps(), schedule=temperature_schedule(), pt=pt_runs(),
+        anneal=anneal_geometric())
+    with open(OUT, "w") as f:
+        json.dump(doc, f, indent=0, separators=(",", ":"))
+    print("wrote", OUT, os.path.getsize(OUT), "bytes")
+
+
+if __name__ == "__main__":
+    main()
