@@ -1,0 +1,164 @@
+/*
+ * sa_hip.h -- C ABI of the MI355X (gfx950) multispin +-J Metropolis engine.
+ *
+ * A "ladder" is a set of `nsys` systems (replica lattices) of one geometry that
+ * live in HBM and share the couplings J and the sweep random stream -- exactly
+ * the temperatures of the reference's parallel tempering, which main_cpu/main_gpu
+ * seed identically (src/main_cpu.cpp:267-296), or a single annealed system
+ * (nsys = 1).  Every system holds one uint64 word per site; bit k of a word is
+ * lane k, one of 64 independent disorder samples (spin_t, src/gpu_sys.cuh:10).
+ *
+ * Results are bit-exact against the reference CPU classes (cpu_2dp, cpu_2d,
+ * cpu_3d): same seeds and random words give the same spin words, energies and
+ * parallel-tempering swap decisions.  Each entry point names the reference
+ * interface it replaces (file:line relative to the reference repository).
+ *
+ * Conventions: every function returns SA_OK (0) or a negative sa_status; a
+ * failing call leaves the ladder unchanged and sa_last_error() describes the
+ * cause.  Host buffers are caller-allocated.  Calls on different ladders may come
+ * from different threads (each ladder owns a HIP stream and no device-global
+ * mutable state exists); calls on one ladder must not overlap.
+ */
+#ifndef SA_HIP_H
+#define SA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sa_ladder sa_ladder;
+
+enum sa_status {
+  SA_OK = 0,
+  SA_ERR_ARG = -1,          /* invalid argument */
+  SA_ERR_UNSUPPORTED = -2,  /* geometry not supported by the device path (odd L) */
+  SA_ERR_HIP = -3,          /* HIP runtime error */
+  SA_ERR_NOMEM = -4,        /* device or pinned allocation failed */
+  SA_ERR_STATE = -5         /* call not valid in the current state */
+};
+
+/* lattice kinds, the reference's -d flag (src/main_cpu.cpp:110-112) */
+enum sa_kind {
+  SA_KIND_2P = 0, /* 2D, open x / periodic y: cpu_2dp, gpu_2d (src/cpu_2dp.cpp:336-383) */
+  SA_KIND_2D = 1, /* 2D fully periodic: cpu_2d (src/cpu_2d.cpp:5-41) */
+  SA_KIND_3D = 2  /* 3D periodic: cpu_3d, gpu_3d (src/cpu_3d.cpp:107-161) */
+};
+
+/* source of the per-site random words of a sweep */
+enum sa_rng_mode {
+  SA_RNG_REPLAY = 0,  /* the reference's own stream: raw mt19937_64 words drawn on the
+                         host in the reference's visit order (src/cpu_3d.cpp:130) */
+  SA_RNG_COUNTER = 1  /* Philox4x32-10 keyed by (seed, sweep, site pair), computed
+                         inline by the sweep kernels */
+};
+
+const char *sa_last_error(void);
+const char *sa_status_string(int status);
+int sa_version(void);
+
+/* Create a ladder of nsys systems (device memory for spins, couplings, tables).
+ * Replaces the gpu_2d/gpu_3d constructors (src/gpu_3d.cu:20-77, src/gpu_2d.cu:20-84).
+ * The device path needs an even L >= 4 (SA_ERR_UNSUPPORTED otherwise). */
+int sa_ladder_create(int kind, int L, int nsys, sa_ladder **out);
+/* Release everything (gpu_free, src/gpu_3d.cu:79-98).  NULL is accepted. */
+int sa_ladder_destroy(sa_ladder *l);
+/* sites per system: L*L or L*L*L (get_N, src/gpu_3d.cu:301-303) */
+int64_t sa_ladder_sites(const sa_ladder *l);
+int sa_ladder_nsys(const sa_ladder *l);
+
+/* Random streams. */
+int sa_ladder_set_rng(sa_ladder *l, int mode);
+/* Re-seed: the host mt19937_64 restarts from `seed` and the counter stream restarts
+ * at sweep 0 (set_seed, src/gpu_sys.cuh:18, CPU src/cpu_3d.cpp:170). */
+int sa_ladder_seed(sa_ladder *l, uint64_t seed);
+/* sweeps done since the last seed (the counter-mode sweep index) */
+int sa_ladder_get_sweep_index(const sa_ladder *l, uint64_t *t);
+int sa_ladder_set_sweep_index(sa_ladder *l, uint64_t t);
+/* the host engine state: 312 words + position, libstdc++ mt19937_64 layout */
+int sa_ladder_get_engine(const sa_ladder *l, uint64_t state[313]);
+int sa_ladder_set_engine(sa_ladder *l, const uint64_t state[313]);
+
+/* Couplings drawn from the host engine exactly as cpu_3d::init_J / cpu_2dp::init_J
+ * (d*N raw words, src/cpu_3d.cpp:281-301, src/cpu_2dp.cpp:243-260); replaces
+ * gpu_3d::init_J (src/gpu_3d.cu:251-271). */
+int sa_ladder_init_couplings(sa_ladder *l);
+/* Spins of every system = the next N raw words (cpu_3d::init_rand,
+ * src/cpu_3d.cpp:257-263); replaces gpu_3d::init_rand (src/gpu_3d.cu:273-279). */
+int sa_ladder_init_spins(sa_ladder *l);
+/* Deduplicated couplings (N words per axis; jz ignored in 2D).  load_J / save_J
+ * (src/gpu_3d.cu:212-249) move the same bonds in the duplicated file layout. */
+int sa_ladder_set_couplings(sa_ladder *l, const uint64_t *jx, const uint64_t *jy, const uint64_t *jz);
+int sa_ladder_get_couplings(const sa_ladder *l, uint64_t *jx, uint64_t *jy, uint64_t *jz);
+/* spins of system `sys` (N words) */
+int sa_ladder_set_spins(sa_ladder *l, int sys, const uint64_t *s);
+int sa_ladder_get_spins(const sa_ladder *l, int sys, uint64_t *s);
+
+/* Temperature and field of one system; computes the reference's Boltzmann table in
+ * double (set_T / set_h, src/gpu_3d.cu:117-166, src/cpu_3d.cpp:218-251) and turns
+ * it into integer acceptance thresholds.  `boltz_out` (may be NULL) receives the
+ * table: 14 doubles (3D) or 20 (2D). */
+int sa_ladder_set_temperature(sa_ladder *l, int sys, double T, double h, double *boltz_out);
+
+/* `nsweeps` sweeps of every system (sweep(), src/gpu_3d.cu:100-114 and
+ * src/cpu_3d.cpp:94-105): per sweep, the offset-0 then the offset-1 sites in the
+ * reference CPU order.  Asynchronous on the ladder's stream in counter mode; in
+ * replay mode the host draws the words and the call returns when they are
+ * consumed. */
+int sa_ladder_sweep(sa_ladder *l, int nsweeps);
+
+/* Per system and lane k (bit k): E[sys*64+k] = sum over the -x,-y[,-z] bond of
+ * every site of +1 (satisfied) / -1, M[sys*64+k] = sum of +1 (set) / -1 (clear).
+ * The reference's measure() returns float(E + h*M) at index k
+ * (src/cpu_3d.cpp:310-371; GPU src/gpu_3d.cu:168-203).  Synchronous. */
+int sa_ladder_measure(sa_ladder *l, int32_t *E, int32_t *M);
+
+/* Exchange the lanes selected by `mask` between systems a and b
+ * (swap, src/gpu_3d.cu:310-328, src/cpu_3d.cpp:429-444). */
+int sa_ladder_swap(sa_ladder *l, int a, int b, uint64_t mask);
+
+/* wait for the ladder's stream */
+int sa_ladder_sync(sa_ladder *l);
+/* the HIP stream (hipStream_t) the ladder launches on, for event timing */
+void *sa_ladder_stream(sa_ladder *l);
+
+/* ---- drivers (C++ templates of include/sa, exposed for non-C++ callers) ------- */
+
+/* Parallel tempering as main_gpu runs it (src/main_gpu.cu:262-416,
+ * src/parallel_tempering.hpp:54-115): nT systems seeded with `seed`
+ * (set_seed, init_J, set_seed, init_rand), temperatures sorted ascending,
+ * `nsteps` x step(0) with `sweeps` sweeps per step, swap decisions on the host from
+ * mt19937(123).  One ladder holds all temperatures.  Outputs may be NULL:
+ * pairs[nsteps], us[nsteps], masks[nsteps], E_last[nT*64] (measure() vectors after
+ * the last step), spins[nT*N]. */
+int sa_pt_run(int kind, int L, int nT, const double *T, double h, uint64_t seed, int sweeps, int nsteps,
+              int rng_mode, int32_t *pairs, double *us, uint64_t *masks, float *E_last, uint64_t *spins);
+
+/* Annealing over a per-sweep temperature schedule (annealing<gpu_sys>,
+ * src/annealing.hpp:67-100, with set_T before every sweep): records measure()
+ * after sweep k when k % stride == 0.  Returns the number of records (>= 0) or an
+ * sa_status. */
+int sa_anneal_run(int kind, int L, const double *Ts, int nsweeps, double h, uint64_t seed, int stride,
+                  int rng_mode, float *E_log, uint64_t *spins_out);
+
+/* A parallel-tempering run kept alive between calls (what sa_pt_run does in one
+ * call), for callers that time or interleave steps: create seeds the systems,
+ * sa_pt_step runs nsteps x step(0), sa_pt_result copies the swap log of the
+ * steps run so far (pairs/us/masks, each capacity entries) and the current
+ * energies and spins. */
+typedef struct sa_pt sa_pt;
+int sa_pt_create(int kind, int L, int nT, const double *T, double h, uint64_t seed, int sweeps, int rng_mode,
+                 sa_pt **out);
+int sa_pt_step(sa_pt *pt, int nsteps);
+int sa_pt_sync(sa_pt *pt);
+int64_t sa_pt_steps_done(const sa_pt *pt);
+int sa_pt_result(sa_pt *pt, int64_t capacity, int32_t *pairs, double *us, uint64_t *masks, float *E_last,
+                 uint64_t *spins);
+sa_ladder *sa_pt_ladder(sa_pt *pt);
+int sa_pt_destroy(sa_pt *pt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,451 @@
+// sa_ladder.hip -- the C ABI of include/sa_hip.h: device-resident ladders and the
+// launches of the sweep, measure and swap kernels (sa_kernels.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sa_hip.h"
+#include "sa_kernels.hpp"
+#include "sa/rng.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return status;
+}
+
+#define SA_HIP_TRY(expr)                                                                  \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) return fail(SA_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int64_t kReplayWordsCap = int64_t(1) << 24;  // 128 MiB of staged random words
+
+}  // namespace
+
+struct sa_ladder {
+  int kind = 0, L = 0, nsys = 0, dim = 0;
+  int64_t N = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t *s = nullptr, *jx = nullptr, *jy = nullptr, *jz = nullptr;
+  sa::sys_table *tab = nullptr;
+  int *cnt = nullptr;
+  std::vector<sa::sys_table> htab;
+  std::vector<char> fast;  // per system: symmetric tables, low levels always accept
+  int rng_mode = SA_RNG_REPLAY;
+  uint64_t seed = 0, t = 0;
+  sa::mt19937_64 eng{};
+  uint64_t *words_d = nullptr, *words_h = nullptr;
+  int64_t words_cap = 0;  // sweeps that fit in the staging buffers
+  int *cnt_h = nullptr;
+};
+
+namespace {
+
+int bonds_per_site(int kind) { return kind == SA_KIND_3D ? 3 : 2; }
+
+void free_all(sa_ladder *l) {
+  if (l->s) (void)hipFree(l->s);
+  if (l->jx) (void)hipFree(l->jx);
+  if (l->jy) (void)hipFree(l->jy);
+  if (l->jz) (void)hipFree(l->jz);
+  if (l->tab) (void)hipFree(l->tab);
+  if (l->cnt) (void)hipFree(l->cnt);
+  if (l->words_d) (void)hipFree(l->words_d);
+  if (l->words_h) (void)hipHostFree(l->words_h);
+  if (l->cnt_h) (void)hipHostFree(l->cnt_h);
+  if (l->stream) (void)hipStreamDestroy(l->stream);
+}
+
+int check_sys(const sa_ladder *l, int sys) {
+  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  if (sys < 0 || sys >= l->nsys) return fail(SA_ERR_ARG, "system index %d out of range [0,%d)", sys, l->nsys);
+  return SA_OK;
+}
+
+int ensure_words(sa_ladder *l, int64_t sweeps) {
+  if (l->words_cap >= sweeps) return SA_OK;
+  if (l->words_d) (void)hipFree(l->words_d);
+  if (l->words_h) (void)hipHostFree(l->words_h);
+  l->words_d = nullptr;
+  l->words_h = nullptr;
+  l->words_cap = 0;
+  if (hipMalloc(&l->words_d, sweeps * l->N * 8) != hipSuccess) return fail(SA_ERR_NOMEM, "replay buffer");
+  if (hipHostMalloc(&l->words_h, sweeps * l->N * 8, hipHostMallocDefault) != hipSuccess)
+    return fail(SA_ERR_NOMEM, "pinned replay buffer");
+  l->words_cap = sweeps;
+  return SA_OK;
+}
+
+template <bool COUNTER, bool FAST, int MB>
+void launch3d(const sa::sweep_args &a, int color, hipStream_t st) {
+  const int cols = a.L * (a.L / 2);
+  dim3 grid((cols + 255) / 256, a.nsys / MB);
+  hipLaunchKernelGGL((sa::k_sweep3d_half<COUNTER, FAST, MB>), grid, dim3(256), 0, st, a, color);
+}
+
+template <bool COUNTER, bool FAST, bool OPEN, int MB>
+void launch2d(const sa::sweep_args &a, int color, hipStream_t st) {
+  const int n = a.L * (a.L / 2);
+  dim3 grid((n + 255) / 256, a.nsys / MB);
+  hipLaunchKernelGGL((sa::k_sweep2d_half<COUNTER, FAST, OPEN, MB>), grid, dim3(256), 0, st, a, color);
+}
+
+template <bool COUNTER, bool FAST>
+void launch_half(const sa_ladder *l, const sa::sweep_args &a, int color) {
+  const int mb = (l->nsys % 4 == 0) ? 4 : (l->nsys % 2 == 0) ? 2 : 1;
+  if (l->kind == SA_KIND_3D) {
+    if (mb == 4) launch3d<COUNTER, FAST, 4>(a, color, l->stream);
+    else if (mb == 2) launch3d<COUNTER, FAST, 2>(a, color, l->stream);
+    else launch3d<COUNTER, FAST, 1>(a, color, l->stream);
+  } else if (l->kind == SA_KIND_2D) {
+    if (mb == 4) launch2d<COUNTER, FAST, false, 4>(a, color, l->stream);
+    else if (mb == 2) launch2d<COUNTER, FAST, false, 2>(a, color, l->stream);
+    else launch2d<COUNTER, FAST, false, 1>(a, color, l->stream);
+  } else {
+    if (mb == 4) launch2d<COUNTER, false, true, 4>(a, color, l->stream);
+    else if (mb == 2) launch2d<COUNTER, false, true, 2>(a, color, l->stream);
+    else launch2d<COUNTER, false, true, 1>(a, color, l->stream);
+  }
+}
+
+void launch_sweep(const sa_ladder *l, const uint64_t *words, uint64_t t) {
+  sa::sweep_args a;
+  a.s = l->s;
+  a.jx = l->jx;
+  a.jy = l->jy;
+  a.jz = l->jz;
+  a.tab = l->tab;
+  a.words = words;
+  a.seed = l->seed;
+  a.sweep = t;
+  a.N = l->N;
+  a.L = l->L;
+  a.nsys = l->nsys;
+  bool fast = true;
+  for (char f : l->fast) fast = fast && f;
+  for (int color = 0; color < 2; ++color) {
+    if (l->rng_mode == SA_RNG_COUNTER) {
+      if (fast) launch_half<true, true>(l, a, color); else launch_half<true, false>(l, a, color);
+    } else {
+      if (fast) launch_half<false, true>(l, a, color); else launch_half<false, false>(l, a, color);
+    }
+  }
+}
+
+// thresholds of one Boltzmann table into sys_table t (levels 0..nb)
+void fill_table(sa::sys_table &t, int tb, const double *b, int nlev) {
+  t.always[tb] = 0;
+  for (int c = 0; c < 8; ++c) t.thr[tb][c] = 0;
+  for (int c = 0; c < nlev; ++c) {
+    const sa::threshold th = sa::accept_threshold(b[c]);
+    t.thr[tb][c] = th.value;
+    if (th.always) t.always[tb] |= 1u << c;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sa_last_error(void) { return g_last_error.c_str(); }
+
+const char *sa_status_string(int status) {
+  switch (status) {
+    case SA_OK: return "ok";
+    case SA_ERR_ARG: return "invalid argument";
+    case SA_ERR_UNSUPPORTED: return "unsupported geometry";
+    case SA_ERR_HIP: return "HIP runtime error";
+    case SA_ERR_NOMEM: return "out of memory";
+    case SA_ERR_STATE: return "invalid state";
+    default: return "unknown status";
+  }
+}
+
+int sa_version(void) { return 1; }
+
+int sa_ladder_create(int kind, int L, int nsys, sa_ladder **out) {
+  if (!out) return fail(SA_ERR_ARG, "null output handle");
+  *out = nullptr;
+  if (kind < SA_KIND_2P || kind > SA_KIND_3D) return fail(SA_ERR_ARG, "unknown kind %d", kind);
+  if (nsys < 1) return fail(SA_ERR_ARG, "nsys must be >= 1");
+  if (L < 4 || (L & 1)) return fail(SA_ERR_UNSUPPORTED, "device path needs an even L >= 4 (got %d)", L);
+  if ((kind == SA_KIND_3D && L > 1024) || (kind != SA_KIND_3D && L > 32768))
+    return fail(SA_ERR_ARG, "L=%d too large", L);
+  sa_ladder *l = new sa_ladder();
+  l->kind = kind;
+  l->L = L;
+  l->nsys = nsys;
+  l->dim = kind == SA_KIND_3D ? 3 : 2;
+  l->N = (int64_t)L * L * (kind == SA_KIND_3D ? L : 1);
+  const int64_t N = l->N;
+  int rc = SA_OK;
+  auto bad = [&](hipError_t e, const char *what) {
+    if (e == hipSuccess) return false;
+    rc = (e == hipErrorOutOfMemory) ? fail(SA_ERR_NOMEM, "%s: %s", what, hipGetErrorString(e))
+                                    : fail(SA_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    return true;
+  };
+  if (bad(hipGetDevice(&l->device), "hipGetDevice") || bad(hipStreamCreate(&l->stream), "hipStreamCreate") ||
+      bad(hipMalloc(&l->s, (size_t)nsys * N * 8), "spins") || bad(hipMalloc(&l->jx, N * 8), "jx") ||
+      bad(hipMalloc(&l->jy, N * 8), "jy") || bad(hipMalloc(&l->jz, N * 8), "jz") ||
+      bad(hipMalloc(&l->tab, sizeof(sa::sys_table) * nsys), "tables") ||
+      bad(hipMalloc(&l->cnt, sizeof(int) * 128 * nsys), "counters") ||
+      bad(hipHostMalloc(&l->cnt_h, sizeof(int) * 128 * nsys, hipHostMallocDefault), "pinned counters") ||
+      bad(hipMemsetAsync(l->s, 0, (size_t)nsys * N * 8, l->stream), "memset") ||
+      bad(hipMemsetAsync(l->jx, 0, N * 8, l->stream), "memset") ||
+      bad(hipMemsetAsync(l->jy, 0, N * 8, l->stream), "memset") ||
+      bad(hipMemsetAsync(l->jz, 0, N * 8, l->stream), "memset")) {
+    free_all(l);
+    delete l;
+    return rc;
+  }
+  l->htab.assign(nsys, sa::sys_table{});
+  l->fast.assign(nsys, 0);
+  l->eng.seed(5489u);
+  *out = l;
+  for (int i = 0; i < nsys; ++i) {
+    rc = sa_ladder_set_temperature(l, i, 1.0, 0.0, nullptr);
+    if (rc != SA_OK) {
+      sa_ladder_destroy(l);
+      *out = nullptr;
+      return rc;
+    }
+  }
+  return SA_OK;
+}
+
+int sa_ladder_destroy(sa_ladder *l) {
+  if (!l) return SA_OK;
+  if (l->stream) (void)hipStreamSynchronize(l->stream);
+  free_all(l);
+  delete l;
+  return SA_OK;
+}
+
+int64_t sa_ladder_sites(const sa_ladder *l) { return l ? l->N : 0; }
+int sa_ladder_nsys(const sa_ladder *l) { return l ? l->nsys : 0; }
+void *sa_ladder_stream(sa_ladder *l) { return l ? (void *)l->stream : nullptr; }
+
+int sa_ladder_set_rng(sa_ladder *l, int mode) {
+  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  if (mode != SA_RNG_REPLAY && mode != SA_RNG_COUNTER) return fail(SA_ERR_ARG, "unknown rng mode %d", mode);
+  l->rng_mode = mode;
+  return SA_OK;
+}
+
+int sa_ladder_seed(sa_ladder *l, uint64_t seed) {
+  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  l->eng.seed(seed);
+  l->seed = seed;
+  l->t = 0;
+  return SA_OK;
+}
+
+int sa_ladder_get_sweep_index(const sa_ladder *l, uint64_t *t) {
+  if (!l || !t) return fail(SA_ERR_ARG, "null argument");
+  *t = l->t;
+  return SA_OK;
+}
+
+int sa_ladder_set_sweep_index(sa_ladder *l, uint64_t t) {
+  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  l->t = t;
+  return SA_OK;
+}
+
+int sa_ladder_get_engine(const sa_ladder *l, uint64_t state[313]) {
+  if (!l || !state) return fail(SA_ERR_ARG, "null argument");
+  std::memcpy(state, l->eng.x, 312 * 8);
+  state[312] = l->eng.p;
+  return SA_OK;
+}
+
+int sa_ladder_set_engine(sa_ladder *l, const uint64_t state[313]) {
+  if (!l || !state) return fail(SA_ERR_ARG, "null argument");
+  if (state[312] > 312) return fail(SA_ERR_ARG, "engine position %llu > 312", (unsigned long long)state[312]);
+  std::memcpy(l->eng.x, state, 312 * 8);
+  l->eng.p = state[312];
+  return SA_OK;
+}
+
+int sa_ladder_set_couplings(sa_ladder *l, const uint64_t *jx, const uint64_t *jy, const uint64_t *jz) {
+  if (!l || !jx || !jy || (l->kind == SA_KIND_3D && !jz)) return fail(SA_ERR_ARG, "null argument");
+  const size_t bytes = l->N * 8;
+  SA_HIP_TRY(hipMemcpyAsync(l->jx, jx, bytes, hipMemcpyHostToDevice, l->stream));
+  SA_HIP_TRY(hipMemcpyAsync(l->jy, jy, bytes, hipMemcpyHostToDevice, l->stream));
+  if (l->kind == SA_KIND_3D) SA_HIP_TRY(hipMemcpyAsync(l->jz, jz, bytes, hipMemcpyHostToDevice, l->stream));
+  SA_HIP_TRY(hipStreamSynchronize(l->stream));
+  return SA_OK;
+}
+
+int sa_ladder_get_couplings(const sa_ladder *l, uint64_t *jx, uint64_t *jy, uint64_t *jz) {
+  if (!l || !jx || !jy || (l->kind == SA_KIND_3D && !jz)) return fail(SA_ERR_ARG, "null argument");
+  const size_t bytes = l->N * 8;
+  SA_HIP_TRY(hipMemcpyAsync(jx, l->jx, bytes, hipMemcpyDeviceToHost, l->stream));
+  SA_HIP_TRY(hipMemcpyAsync(jy, l->jy, bytes, hipMemcpyDeviceToHost, l->stream));
+  if (l->kind == SA_KIND_3D) SA_HIP_TRY(hipMemcpyAsync(jz, l->jz, bytes, hipMemcpyDeviceToHost, l->stream));
+  SA_HIP_TRY(hipStreamSynchronize(l->stream));
+  return SA_OK;
+}
+
+int sa_ladder_init_couplings(sa_ladder *l) {
+  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  const int64_t N = l->N;
+  const int d = bonds_per_site(l->kind);
+  std::vector<uint64_t> jx(N), jy(N), jz(l->kind == SA_KIND_3D ? N : 1);
+  // the reference draws d*N words into buffer[d*i + axis] (src/cpu_3d.cpp:285-300)
+  for (int64_t i = 0; i < N; ++i) {
+    jx[i] = l->eng();
+    jy[i] = l->eng();
+    if (d == 3) jz[i] = l->eng();
+  }
+  return sa_ladder_set_couplings(l, jx.data(), jy.data(), d == 3 ? jz.data() : nullptr);
+}
+
+int sa_ladder_init_spins(sa_ladder *l) {
+  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  std::vector<uint64_t> s(l->N);
+  l->eng.fill(s.data(), l->N);
+  for (int i = 0; i < l->nsys; ++i)
+    SA_HIP_TRY(hipMemcpyAsync(l->s + (int64_t)i * l->N, s.data(), l->N * 8, hipMemcpyHostToDevice, l->stream));
+  SA_HIP_TRY(hipStreamSynchronize(l->stream));
+  return SA_OK;
+}
+
+int sa_ladder_set_spins(sa_ladder *l, int sys, const uint64_t *s) {
+  int rc = check_sys(l, sys);
+  if (rc) return rc;
+  if (!s) return fail(SA_ERR_ARG, "null spins");
+  SA_HIP_TRY(hipMemcpyAsync(l->s + (int64_t)sys * l->N, s, l->N * 8, hipMemcpyHostToDevice, l->stream));
+  SA_HIP_TRY(hipStreamSynchronize(l->stream));
+  return SA_OK;
+}
+
+int sa_ladder_get_spins(const sa_ladder *l, int sys, uint64_t *s) {
+  int rc = check_sys(l, sys);
+  if (rc) return rc;
+  if (!s) return fail(SA_ERR_ARG, "null spins");
+  SA_HIP_TRY(hipMemcpyAsync(s, l->s + (int64_t)sys * l->N, l->N * 8, hipMemcpyDeviceToHost, l->stream));
+  SA_HIP_TRY(hipStreamSynchronize(l->stream));
+  return SA_OK;
+}
+
+int sa_ladder_set_temperature(sa_ladder *l, int sys, double T, double h, double *boltz_out) {
+  int rc = check_sys(l, sys);
+  if (rc) return rc;
+  double b[20];
+  sa::sys_table t{};
+  const int nb = l->kind == SA_KIND_3D ? 6 : 4;
+  if (l->kind == SA_KIND_3D) {
+    sa::boltz_3d(T, h, b);
+    fill_table(t, 0, b, 7);
+    fill_table(t, 1, b + 7, 7);
+  } else {
+    sa::boltz_2d(T, h, b);
+    fill_table(t, 0, b, 5);
+    fill_table(t, 1, b + 5, 5);
+    fill_table(t, 2, b + 10, 5);
+    fill_table(t, 3, b + 15, 5);
+  }
+  if (boltz_out) std::memcpy(boltz_out, b, sizeof(double) * (l->kind == SA_KIND_3D ? 14 : 20));
+  // fast kernels: identical spin tables, levels 0..nb/2 always accepted
+  bool fast = l->kind != SA_KIND_2P && std::memcmp(t.thr[0], t.thr[1], sizeof t.thr[0]) == 0 &&
+              t.always[0] == t.always[1];
+  const uint32_t low = (1u << (nb / 2 + 1)) - 1u;
+  fast = fast && (t.always[0] & low) == low && (t.always[0] >> (nb / 2 + 1)) == 0;
+  l->htab[sys] = t;
+  l->fast[sys] = fast;
+  SA_HIP_TRY(hipMemcpyAsync(l->tab + sys, &l->htab[sys], sizeof(sa::sys_table), hipMemcpyHostToDevice, l->stream));
+  SA_HIP_TRY(hipStreamSynchronize(l->stream));
+  return SA_OK;
+}
+
+int sa_ladder_sweep(sa_ladder *l, int nsweeps) {
+  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  if (nsweeps < 0) return fail(SA_ERR_ARG, "negative sweep count");
+  if (nsweeps == 0) return SA_OK;
+  if (l->rng_mode == SA_RNG_COUNTER) {
+    for (int k = 0; k < nsweeps; ++k) launch_sweep(l, nullptr, l->t + k);
+    SA_HIP_TRY(hipGetLastError());
+    l->t += nsweeps;
+    return SA_OK;
+  }
+  const int64_t per_chunk = std::max<int64_t>(1, std::min<int64_t>(nsweeps, kReplayWordsCap / l->N));
+  int rc = ensure_words(l, per_chunk);
+  if (rc) return rc;
+  for (int64_t done = 0; done < nsweeps;) {
+    const int64_t n = std::min<int64_t>(per_chunk, nsweeps - done);
+    l->eng.fill(l->words_h, n * l->N);
+    SA_HIP_TRY(hipMemcpyAsync(l->words_d, l->words_h, n * l->N * 8, hipMemcpyHostToDevice, l->stream));
+    for (int64_t k = 0; k < n; ++k) launch_sweep(l, l->words_d + k * l->N, l->t + done + k);
+    SA_HIP_TRY(hipGetLastError());
+    SA_HIP_TRY(hipStreamSynchronize(l->stream));
+    done += n;
+  }
+  l->t += nsweeps;
+  return SA_OK;
+}
+
+int sa_ladder_measure(sa_ladder *l, int32_t *E, int32_t *M) {
+  if (!l || !E || !M) return fail(SA_ERR_ARG, "null argument");
+  const int n = 128 * l->nsys;
+  hipLaunchKernelGGL(sa::k_fill_zero, dim3((n + 255) / 256), dim3(256), 0, l->stream, l->cnt, n);
+  const int64_t blocks = std::min<int64_t>((l->N + 255) / 256, 1024);
+  dim3 grid((unsigned)blocks, l->nsys);
+  if (l->kind == SA_KIND_3D)
+    hipLaunchKernelGGL(sa::k_measure<sa::KIND_3D>, grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, l->L,
+                       l->N, l->cnt);
+  else if (l->kind == SA_KIND_2D)
+    hipLaunchKernelGGL(sa::k_measure<sa::KIND_2D>, grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, l->L,
+                       l->N, l->cnt);
+  else
+    hipLaunchKernelGGL(sa::k_measure<sa::KIND_2P>, grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, l->L,
+                       l->N, l->cnt);
+  SA_HIP_TRY(hipGetLastError());
+  SA_HIP_TRY(hipMemcpyAsync(l->cnt_h, l->cnt, sizeof(int) * n, hipMemcpyDeviceToHost, l->stream));
+  SA_HIP_TRY(hipStreamSynchronize(l->stream));
+  const int64_t nb = bonds_per_site(l->kind) * l->N - (l->kind == SA_KIND_2P ? l->L : 0);
+  for (int i = 0; i < l->nsys; ++i)
+    for (int k = 0; k < 64; ++k) {
+      E[i * 64 + k] = (int32_t)(2 * (int64_t)l->cnt_h[(2 * i) * 64 + k] - nb);
+      M[i * 64 + k] = (int32_t)(2 * (int64_t)l->cnt_h[(2 * i + 1) * 64 + k] - l->N);
+    }
+  return SA_OK;
+}
+
+int sa_ladder_swap(sa_ladder *l, int a, int b, uint64_t mask) {
+  int rc = check_sys(l, a);
+  if (rc) return rc;
+  rc = check_sys(l, b);
+  if (rc) return rc;
+  if (a == b || mask == 0) return SA_OK;
+  const int64_t blocks = std::min<int64_t>((l->N + 255) / 256, 2048);
+  hipLaunchKernelGGL(sa::k_swap, dim3((unsigned)blocks), dim3(256), 0, l->stream, l->s + (int64_t)a * l->N,
+                     l->s + (int64_t)b * l->N, mask, l->N);
+  SA_HIP_TRY(hipGetLastError());
+  return SA_OK;
+}
+
+int sa_ladder_sync(sa_ladder *l) {
+  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  SA_HIP_TRY(hipStreamSynchronize(l->stream));
+  return SA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,238 @@
+"""Parity of the HIP path (through the C ABI) with the reference.
+
+Replay mode (the reference's own mt19937_64 stream) is checked against the
+golden fixtures written from the compiled reference; counter mode (Philox words)
+against the oracle fed the same words.  Bar: bit-exact spin words, integer
+energies and swap decisions."""
+import numpy as np
+import pytest
+
+from helpers import f32, f64, sha, words
+
+pytestmark = pytest.mark.gpu
+
+KIND_NAME = {0: "2p", 1: "2d", 2: "3d"}
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def sa():
+    import simulated_annealing_amd as sa
+
+    sa.lib()
+    return sa
+
+
+def device_cases(golden):
+    return [c for c in golden["sweeps"] if c["L"] % 2 == 0 and c["L"] >= 4]
+
+
+# ---- replay mode vs the compiled reference's fixtures ---------------------------------
+def test_replay_sweeps_match_reference_fixtures(sa, golden):
+    for c in device_cases(golden):
+        kind, L = c["kind"], c["L"]
+        say("replay sweeps", KIND_NAME[kind], "L", L, "T", c["T"], "h", c["h"])
+        lad = sa.Ladder(kind, L, 1)
+        lad.set_rng(sa.RNG_REPLAY)
+        lad.seed_like_main(c["seed"])
+        lad.set_temperature(0, c["T"], c["h"])
+        assert sha(lad.spins(0)) == c["s0_sha"]
+        jx, jy, jz = lad.couplings()
+        assert sha(jx) == c["jx_sha"] and sha(jy) == c["jy_sha"]
+        if kind == 2:
+            assert sha(jz) == c["jz_sha"]
+        E, M = lad.measure()
+        got = (E[0].astype(np.float64) + c["h"] * M[0].astype(np.float64)).astype(np.float32)
+        assert (got == f32(c["E0"])).all()
+        for k in range(c["sweeps"]):
+            lad.sweep(1)
+            E, M = lad.measure()
+            got = (E[0].astype(np.float64) + c["h"] * M[0].astype(np.float64)).astype(np.float32)
+            assert (got == f32(c["E"][k])).all(), (c["kind"], L, k)
+        s = lad.spins(0)
+        assert sha(s) == c["s_sha"]
+        if "s" in c:
+            assert (s == words(c["s"])).all()
+        lad.close()
+
+
+def test_replay_multi_sweep_call_matches_single_calls(sa, golden):
+    c = [c for c in device_cases(golden) if c["kind"] == 2][0]
+    say("replay sweep(n) vs n x sweep(1)", c["L"])
+    lad = sa.Ladder(2, c["L"], 1)
+    lad.seed_like_main(c["seed"])
+    lad.set_temperature(0, c["T"], c["h"])
+    lad.sweep(c["sweeps"])
+    assert sha(lad.spins(0)) == c["s_sha"]
+
+
+def test_boltz_tables_match_reference_bits(sa, golden):
+    for t in golden["tables"]:
+        say("boltz", t)
+        lad = sa.Ladder(t["kind"], 4, 1)
+        b = lad.set_temperature(0, t["T"], t["h"])
+        assert (b.view(np.uint64) == f64(t["boltz"]).view(np.uint64)).all()
+        lad.close()
+
+
+def test_schedule_anneal_matches_reference_fixture(sa, golden):
+    c = golden["schedule"]
+    say("anneal schedule", c["kind"], c["L"])
+    E, s = sa.anneal_run(c["kind"], c["L"], np.array(c["T"]), 0.0, c["seed"], c["stride"], sa.RNG_REPLAY)
+    assert len(E) == len(c["E"])
+    for a, b in zip(E, c["E"]):
+        assert (a == f32(b)).all()
+    assert sha(s) == c["s_sha"]
+
+
+def test_pt_replay_matches_reference_fixtures(sa, golden):
+    for c in golden["pt"]:
+        if c["L"] % 2:
+            continue
+        say("pt replay", KIND_NAME[c["kind"]], "L", c["L"], "nT", len(c["T"]), "h", c["h"])
+        r = sa.pt_run(c["kind"], c["L"], np.array(c["T"]), c["h"], c["seed"], c["sweeps"], c["nsteps"],
+                      sa.RNG_REPLAY)
+        assert list(r["pairs"]) == c["pairs"]
+        assert (r["us"] == f64(c["us"])).all()
+        assert (r["masks"] == words(c["masks"])).all()
+        for j, e in enumerate(c["E"]):
+            assert (r["E"][j] == f32(e)).all()
+        assert [sha(s) for s in r["spins"]] == c["spins_sha"]
+
+
+# ---- counter mode vs the oracle fed the same Philox words -------------------------------
+@pytest.mark.parametrize("kind,L,nsys,nsw", [(2, 8, 1, 6), (2, 16, 3, 4), (2, 32, 4, 3), (1, 16, 2, 8),
+                                             (1, 64, 4, 5), (0, 16, 2, 8), (0, 30, 1, 6), (2, 6, 2, 5)])
+def test_counter_sweeps_match_oracle(sa, orc, kind, L, nsys, nsw):
+    say("counter sweeps", KIND_NAME[kind], "L", L, "nsys", nsys, "sweeps", nsw)
+    seed = 777 + L
+    lad = sa.Ladder(kind, L, nsys)
+    lad.set_rng(sa.RNG_COUNTER)
+    lad.seed_like_main(seed)
+    Ts = [0.6 + 0.7 * j for j in range(nsys)]
+    hs = [0.0 if j % 2 == 0 else 0.3 for j in range(nsys)]
+    for j in range(nsys):
+        lad.set_temperature(j, Ts[j], hs[j])
+    jx, jy, jz, s0, _ = orc.init_system(kind, L, seed)
+    ref = [s0.copy() for _ in range(nsys)]
+    for t in range(nsw):
+        lad.sweep(1)
+        w = orc.counter_words(kind, L, seed, t)
+        for j in range(nsys):
+            orc.sweep(kind, L, ref[j], jx, jy, jz, orc.boltz(kind, Ts[j], hs[j]), w)
+        E, M = lad.measure()
+        for j in range(nsys):
+            e, m = orc.measure(kind, L, ref[j], jx, jy, jz)
+            assert (E[j] == e).all() and (M[j] == m).all(), (t, j)
+    for j in range(nsys):
+        assert (lad.spins(j) == ref[j]).all(), j
+    assert lad.sweep_index == nsw
+
+
+def test_counter_3d_L128_two_sweeps_match_oracle(sa, orc):
+    kind, L = 2, 128
+    say("counter 3d L128 two sweeps")
+    lad = sa.Ladder(kind, L, 1)
+    lad.set_rng(sa.RNG_COUNTER)
+    lad.seed_like_main(1234)
+    lad.set_temperature(0, 0.5, 0.0)
+    jx, jy, jz, s, _ = orc.init_system(kind, L, 1234)
+    lad.sweep(2)
+    for t in range(2):
+        orc.sweep(kind, L, s, jx, jy, jz, orc.boltz(kind, 0.5), orc.counter_words(kind, L, 1234, t))
+    assert (lad.spins(0) == s).all()
+
+
+def test_pt_counter_matches_oracle(sa, orc):
+    for kind, L, nT, sweeps, nsteps in [(2, 8, 8, 4, 12), (1, 16, 5, 3, 20), (0, 12, 4, 2, 15)]:
+        say("pt counter", KIND_NAME[kind], L, nT)
+        T = np.array([0.5 * 4 ** (k / (nT - 1)) for k in range(nT)])
+        a = sa.pt_run(kind, L, T, 0.0, 4321, sweeps, nsteps, sa.RNG_COUNTER)
+        b = orc.pt_run(kind, L, T, 0.0, 4321, sweeps, nsteps, rng_mode=1)
+        for k in ("pairs", "us", "masks", "E", "spins"):
+            assert (a[k] == b[k]).all(), k
+
+
+# ---- measure / swap on arbitrary states ------------------------------------------------
+@pytest.mark.parametrize("kind,L", [(0, 20), (1, 20), (2, 10), (2, 64)])
+def test_measure_and_swap_on_random_states(sa, orc, kind, L):
+    say("measure/swap", KIND_NAME[kind], L)
+    rng = np.random.default_rng(L)
+    N = L * L * (L if kind == 2 else 1)
+    r = lambda: rng.integers(0, 2**64, size=N, dtype=np.uint64)  # noqa: E731
+    jx, jy, jz, a, b = r(), r(), r(), r(), r()
+    lad = sa.Ladder(kind, L, 3)
+    lad.set_couplings(jx, jy, jz if kind == 2 else None)
+    lad.set_spins(0, a)
+    lad.set_spins(2, b)
+    E, M = lad.measure()
+    for j, s in ((0, a), (2, b)):
+        e, m = orc.measure(kind, L, s, jx, jy, jz)
+        assert (E[j] == e).all() and (M[j] == m).all()
+    mask = 0xF0F0_0000_FFFF_0001
+    lad.swap(0, 2, mask)
+    orc.swap(a, b, mask)
+    assert (lad.spins(0) == a).all() and (lad.spins(2) == b).all()
+
+
+# ---- size-independent properties at the benchmark size ----------------------------------
+def test_infinite_temperature_flips_every_spin_L128(sa):
+    say("T=inf flips, 3d L128 x 2 systems")
+    lad = sa.Ladder(2, 128, 2)
+    lad.set_rng(sa.RNG_COUNTER)
+    lad.seed_like_main(5)
+    lad.set_temperature(0, float("inf"))
+    lad.set_temperature(1, float("inf"))
+    s0 = lad.spins(1)
+    lad.sweep(1)
+    assert (lad.spins(1) == ~s0).all()
+    lad.sweep(1)
+    assert (lad.spins(1) == s0).all()
+
+
+def test_ladder_slot_equals_lone_system_L128(sa):
+    """A system's result does not depend on the other slots of its ladder."""
+    say("ladder slot vs alone, 3d L128")
+    T = [0.5, 0.9, 1.4, 2.0]
+    big = sa.Ladder(2, 128, 4)
+    big.set_rng(sa.RNG_COUNTER)
+    big.seed_like_main(99)
+    for j, t in enumerate(T):
+        big.set_temperature(j, t)
+    big.sweep(3)
+    lone = sa.Ladder(2, 128, 1)
+    lone.set_rng(sa.RNG_COUNTER)
+    lone.seed_like_main(99)
+    lone.set_temperature(0, T[2])
+    lone.sweep(3)
+    assert (big.spins(2) == lone.spins(0)).all()
+    E1, M1 = big.measure()
+    E2, M2 = lone.measure()
+    assert (E1[2] == E2[0]).all() and (M1[2] == M2[0]).all()
+
+
+def test_ferromagnetic_ground_state_L128(sa):
+    say("ground state energy 3d L128")
+    L = 128
+    N = L**3
+    one = np.full(N, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
+    lad = sa.Ladder(2, L, 1)
+    lad.set_couplings(one, one, one)
+    lad.set_spins(0, np.zeros(N, dtype=np.uint64))
+    E, M = lad.measure()
+    assert (E == 3 * N).all() and (M == -N).all()
+    # at T -> 0 a ground state stays put
+    lad.set_temperature(0, 1e-3)
+    lad.set_rng(sa.RNG_COUNTER)
+    lad.sweep(2)
+    assert (lad.spins(0) == 0).all()
+
+
+def test_unsupported_geometry_is_an_error(sa):
+    with pytest.raises(sa.SAError):
+        sa.Ladder(2, 7, 1)
+    with pytest.raises(sa.SAError):
+        sa.Ladder(1, 2, 1)
