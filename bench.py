@@ -1,0 +1,178 @@
+"""Benchmark: 3D +-J Edwards-Anderson spin glass, L=128, parallel tempering over 32
+temperatures geometric in [0.5, 2.0], 10 sweeps between replica-exchange attempts,
+64 disorder samples per uint64 site word, on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 500] [--warmup 5]
+
+A step is one parallel-tempering step of the whole ladder: 10 sweeps of all 32
+temperatures (counter RNG), the energy measurement and one exchange attempt
+decided on the host.  The default 500 steps are the full 5,000-sweep workload.
+Prints one JSON line: replica spin-flip attempts/ns, the roofline of the dominant
+kernel (HIP events on its stream) and a CPU baseline (the reference's own CPU
+code, compiled from its sources, timed on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+KIND, L, NT, SWEEPS = 2, 128, 32, 10
+T_LADDER = [0.5 * 4.0 ** (k / (NT - 1)) for k in range(NT)]
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweep3d_L128_pt32_traffic.json")
+
+
+class HipEvents:
+    """hipEvent timing on a given stream (libamdhip64 through ctypes)."""
+
+    def __init__(self):
+        self.h = C.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.h.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.h.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.h.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+
+def cpu_baseline():
+    """The reference's parallel_tempering<cpu_sys> over cpu_3d (compiled from the
+    reference sources into oracle/_ref), OpenMP over the temperatures as in the
+    reference, on this host's cores: one step of 2 sweeps of the same ladder."""
+    import oracle
+
+    cores = len(os.sched_getaffinity(0))
+    sweeps = 2
+    if oracle.ref_available():
+        R = oracle.ref()
+        secs = R.ref_time_pt(KIND, L, NT, np.array(T_LADDER), 1234, sweeps, 1, cores)
+        kind = "reference"
+        sample = f"1 PT step x {sweeps} sweeps of the 32-temperature L=128 ladder (reference cpu_3d, OpenMP)"
+    else:  # the oracle restatement, one system on one core
+        cores = 1
+        sweeps = 1
+        jx, jy, jz, s, g = oracle.init_system(KIND, L, 1234)
+        tab = oracle.boltz(KIND, 1.0)
+        w = g.fill(s.size)
+        t0 = time.perf_counter()
+        oracle.sweep(KIND, L, s, jx, jy, jz, tab, w)
+        secs = time.perf_counter() - t0
+        kind = "port"
+        sample = "1 sweep of one L=128 system (oracle restatement, 1 core)"
+        return dict(value=sweeps * L**3 * 64 / secs / 1e9, unit="attempts/ns", cores=cores, kind=kind,
+                    sample=sample)
+    attempts = NT * sweeps * L**3 * 64
+    return dict(value=attempts / secs / 1e9, unit="attempts/ns", cores=cores, kind=kind, sample=sample)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only the masked-swap kernel on two 128 MiB systems (known traffic: 256 MiB read, "
+                         "256 MiB written per launch) to scale the FETCH_SIZE/WRITE_SIZE counters")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        import simulated_annealing_amd as sa
+
+        lad = sa.Ladder(KIND, 256, 2)
+        for _ in range(5):
+            lad.swap(0, 1, 0xFFFFFFFFFFFFFFFF)
+        lad.sync()
+        print(json.dumps({"calibration": "k_swap", "bytes_read_per_launch": 2 * 256**3 * 8,
+                          "bytes_written_per_launch": 2 * 256**3 * 8, "launches": 5}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(independent ladders shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import simulated_annealing_amd as sa
+
+    pt = sa.PT(KIND, L, T_LADDER, 0.0, 1234, SWEEPS, sa.RNG_COUNTER)
+    pt.step(args.warmup)
+    pt.sync()
+    t0 = time.perf_counter()
+    pt.step(args.steps)
+    pt.sync()
+    dt = time.perf_counter() - t0
+    N = L**3
+    attempts = args.steps * SWEEPS * NT * N * 64
+    value = attempts / dt / 1e9
+
+    # dominant kernel: the 3D half-sweep over the whole ladder.  Time back-to-back
+    # sweeps with hipEvents on the ladder's stream.
+    lad = pt.ladder_handle()
+    stream = sa.lib().sa_ladder_stream(lad)
+    ev = HipEvents()
+    a, b = ev.event(), ev.event()
+    nsw = 40
+    ev.record(a, stream)
+    sa.check(sa.lib().sa_ladder_sweep(lad, nsw), "sweep")
+    ev.record(b, stream)
+    ms = ev.elapsed_ms(a, b)
+    launch_us = ms * 1e3 / (2 * nsw)
+    # algorithmic bytes per half-sweep launch: every spin word of the 32 systems read
+    # and written once per sweep (16 B per word-sweep) plus each coupling word once
+    # per sweep for the whole ladder (24 B per site, shared): half of it per launch.
+    alg_bytes = 0.5 * (NT * N * 16 + N * 24)
+    achieved = alg_bytes / (launch_us * 1e-6) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                    kernel="k_sweep3d_half", launch_us=round(launch_us, 2), alg_bytes_per_launch=alg_bytes)
+    cpu = None if args.no_cpu else cpu_baseline()
+    line = {
+        "metric": "replica spin-flip attempts/ns (1 GPU, 3D L=128)",
+        "value": round(value, 1),
+        "unit": "attempts/ns",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "3D +-J EA L=128 parallel tempering, 32 temperatures geometric 0.5..2.0, "
+                               "10 sweeps per exchange, 64 samples per word",
+                   "L": L, "temperatures": NT, "sweeps_per_step": SWEEPS, "rng": "counter (Philox4x32-10)",
+                   "sweeps_timed": args.steps * SWEEPS},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    pt.close()
+
+
+if __name__ == "__main__":
+    main()

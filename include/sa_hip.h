@@ -118,6 +118,13 @@ int sa_ladder_measure(sa_ladder *l, int32_t *E, int32_t *M);
  * (swap, src/gpu_3d.cu:310-328, src/cpu_3d.cpp:429-444). */
 int sa_ladder_swap(sa_ladder *l, int a, int b, uint64_t mask);
 
+/* Kernel tuning (no effect on results).  SA_TUNE_SYS_PER_THREAD: systems one
+ * thread updates per site (1, 2 or 4; it must divide nsys; 0 = automatic) -- more
+ * share the coupling loads and the random word, fewer give more threads. */
+enum sa_tune_key { SA_TUNE_SYS_PER_THREAD = 1 };
+int sa_ladder_set_tuning(sa_ladder *l, int key, int value);
+int sa_ladder_get_tuning(const sa_ladder *l, int key, int *value);
+
 /* wait for the ladder's stream */
 int sa_ladder_sync(sa_ladder *l);
 /* the HIP stream (hipStream_t) the ladder launches on, for event timing */

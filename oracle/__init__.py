@@ -272,6 +272,11 @@ def ref():
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         R.ref_anneal_geometric.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                            C.c_double, C.c_long, C.c_char_p, C.c_long]
+        R.ref_time_sweeps.argtypes = [C.c_int, C.c_int, C.c_double, C.c_long, C.c_int]
+        R.ref_time_sweeps.restype = C.c_double
+        R.ref_time_pt.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, C.c_long, C.c_int, C.c_int, C.c_int]
+        R.ref_time_pt.restype = C.c_double
+        R.ref_omp_max_threads.restype = C.c_int
         _ref = R
     return _ref
 

@@ -19,6 +19,8 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <chrono>
+#include <omp.h>
 
 #define private public
 #define protected public
@@ -198,5 +200,53 @@ int ref_anneal_geometric(int kind, int L, double T0, double alfa, int sweeps, in
   std::memcpy(buf, s.c_str(), s.size() + 1);
   return (int)s.size();
 }
+
+// Timing of the reference's own code, for the CPU baseline: seconds spent in
+// `nsweeps` cpu_*::sweep() calls of one system (setup excluded), the measurement
+// the reference's -t mode makes (src/main_cpu.cpp:183-195).
+double ref_time_sweeps(int kind, int L, double T, long seed, int nsweeps) {
+  Logger::verbosity = 0;
+  cpu_sys *q = make_sys(kind, L, T, 0.0);
+  q->set_seed(seed);
+  q->init_J();
+  q->set_seed(seed);
+  q->init_rand();
+  auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < nsweeps; ++i) q->sweep();
+  auto t1 = std::chrono::steady_clock::now();
+  delete q;
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// Seconds spent in `nsteps` x parallel_tempering::step(0) (OpenMP over the
+// temperatures, src/parallel_tempering.hpp:86) with `nthreads` threads (0 = the
+// OpenMP default); setup excluded.
+double ref_time_pt(int kind, int L, int nT, const double *Tin, long seed, int sweeps, int nsteps, int nthreads) {
+  Logger::verbosity = 0;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  std::vector<cpu_sys *> sys;
+  for (int i = 0; i < nT; ++i) {
+    cpu_sys *q = make_sys(kind, L, Tin[0], 0.0);
+    q->set_seed(seed);
+    q->init_J();
+    q->set_seed(seed);
+    q->init_rand();
+    sys.push_back(q);
+  }
+  std::vector<double> T(Tin, Tin + nT);
+  std::string fname = "/tmp/sa_ref_harness_%.dat";
+  double dt;
+  {
+    parallel_tempering<cpu_sys> pt(sys, T, sweeps, fname);
+    auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < nsteps; ++k) pt.step(0);
+    auto t1 = std::chrono::steady_clock::now();
+    dt = std::chrono::duration<double>(t1 - t0).count();
+  }
+  for (cpu_sys *q : sys) delete q;
+  return dt;
+}
+
+int ref_omp_max_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(PKG, "libsa_hip.so")
 KIND_2P, KIND_2D, KIND_3D = 0, 1, 2
 RNG_REPLAY, RNG_COUNTER = 0, 1
 SA_OK = 0
+TUNE_SYS_PER_THREAD = 1
 
 # every symbol include/sa_hip.h declares (checked by tests/test_capi_symbols.py)
 EXPORTS = [
@@ -28,6 +29,7 @@ EXPORTS = [
     "sa_ladder_init_couplings", "sa_ladder_init_spins", "sa_ladder_set_couplings", "sa_ladder_get_couplings",
     "sa_ladder_set_spins", "sa_ladder_get_spins", "sa_ladder_set_temperature", "sa_ladder_sweep",
     "sa_ladder_measure", "sa_ladder_swap", "sa_ladder_sync", "sa_ladder_stream",
+    "sa_ladder_set_tuning", "sa_ladder_get_tuning",
     "sa_pt_run", "sa_anneal_run",
     "sa_pt_create", "sa_pt_step", "sa_pt_sync", "sa_pt_steps_done", "sa_pt_result", "sa_pt_ladder",
     "sa_pt_destroy",
@@ -81,6 +83,8 @@ def lib():
         L.sa_ladder_sync.argtypes = [_vp]
         L.sa_ladder_stream.argtypes = [_vp]
         L.sa_ladder_stream.restype = _vp
+        L.sa_ladder_set_tuning.argtypes = [_vp, C.c_int, C.c_int]
+        L.sa_ladder_get_tuning.argtypes = [_vp, C.c_int, C.POINTER(C.c_int)]
         L.sa_pt_run.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, C.c_double, C.c_uint64, C.c_int, C.c_int, C.c_int,
                                 _vp, _vp, _vp, _vp, _vp]
         L.sa_anneal_run.argtypes = [C.c_int, C.c_int, _f64p, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int,
@@ -219,6 +223,14 @@ class Ladder:
 
     def sync(self):
         check(lib().sa_ladder_sync(self.h), "sync")
+
+    def set_sys_per_thread(self, mb: int):
+        check(lib().sa_ladder_set_tuning(self.h, TUNE_SYS_PER_THREAD, mb), "set_tuning")
+
+    def sys_per_thread(self) -> int:
+        v = C.c_int()
+        check(lib().sa_ladder_get_tuning(self.h, TUNE_SYS_PER_THREAD, C.byref(v)), "get_tuning")
+        return v.value
 
 
 class PT:

@@ -130,6 +130,17 @@ __device__ __forceinline__ uint64_t flip_fast2(const sys_table &t, uint64_t g, c
 // ---------------------------------------------------------------- 3D half-sweep
 // Thread = one column (x, y) of colour `color` and MB systems; grid.y = system
 // block (so thresholds are wave-uniform and live in scalar registers).
+//
+// Software pipeline: nothing a step loads depends on the step before it (the
+// in-plane neighbours have the other colour and do not change during the half;
+// the column's own next word is still old), so the loads of step z+1 are issued
+// before the arithmetic and the store of step z and overlap its latency.
+template <int MB>
+struct plane_loads {
+  uint64_t jw, je, jn, js, jd;  // couplings: -x, +x, -y, +y, +z bonds
+  uint64_t w[MB], e[MB], n[MB], s[MB], d[MB];
+};
+
 template <bool COUNTER, bool FAST, int MB>
 __global__ void __launch_bounds__(256) k_sweep3d_half(sweep_args p, int color) {
   const int L = p.L;
@@ -158,32 +169,49 @@ __global__ void __launch_bounds__(256) k_sweep3d_half(sweep_args p, int color) {
     own[m] = sp[m][c0];
     first[m] = 0;
   }
-  uint64_t jz_up = p.jz[c0];
-  u32x4 blk{0, 0, 0, 0};
-  for (int z = 0; z < L; ++z) {
+  // loads of plane z (the +z neighbour of plane L-1 is the new plane 0: not loaded)
+  auto load = [&](int z, plane_loads<MB> &q) {
     const int64_t P = z * L2;
     const int64_t Pd = (z == L - 1) ? 0 : P + L2;
-    const uint64_t jw = p.jx[P + c0], je = p.jx[P + ce];
-    const uint64_t jn = p.jy[P + c0], js = p.jy[P + cs];
-    const uint64_t jd = p.jz[Pd + c0];
+    q.jw = p.jx[P + c0];
+    q.je = p.jx[P + ce];
+    q.jn = p.jy[P + c0];
+    q.js = p.jy[P + cs];
+    q.jd = p.jz[Pd + c0];
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      const uint64_t *s = sp[m];
+      q.w[m] = s[P + cw];
+      q.e[m] = s[P + ce];
+      q.n[m] = s[P + cn];
+      q.s[m] = s[P + cs];
+      q.d[m] = (z == L - 1) ? 0 : s[Pd + c0];
+    }
+  };
+  uint64_t jz_up = p.jz[c0];
+  u32x4 blk{0, 0, 0, 0};
+  plane_loads<MB> cur, nxt;
+  load(0, cur);
+  for (int z = 0; z < L; ++z) {
+    if (z + 1 < L) load(z + 1, nxt);
     uint64_t g;
     if (COUNTER) {
-      if ((z & 1) == 0) blk = counter_block((uint32_t)(c0 + P), p.seed, p.sweep);
+      if ((z & 1) == 0) blk = counter_block((uint32_t)(c0 + z * L2), p.seed, p.sweep);
       g = counter_half(blk, z & 1);
     } else {
       g = p.words[rank0 + z * rstep];
     }
+    const int64_t P = z * L2;
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
-      const uint64_t *s = sp[m];
       const uint64_t si = own[m];
-      const uint64_t down = (z == L - 1) ? first[m] : s[Pd + c0];
-      const uint64_t bw = jw ^ si ^ s[P + cw];
-      const uint64_t be = je ^ si ^ s[P + ce];
-      const uint64_t bn = jn ^ si ^ s[P + cn];
-      const uint64_t bs = js ^ si ^ s[P + cs];
+      const uint64_t down = (z == L - 1) ? first[m] : cur.d[m];
+      const uint64_t bw = cur.jw ^ si ^ cur.w[m];
+      const uint64_t be = cur.je ^ si ^ cur.e[m];
+      const uint64_t bn = cur.jn ^ si ^ cur.n[m];
+      const uint64_t bs = cur.js ^ si ^ cur.s[m];
       const uint64_t bu = jz_up ^ si ^ up[m];
-      const uint64_t bd = jd ^ si ^ down;
+      const uint64_t bd = cur.jd ^ si ^ down;
       const count3 c = count6(bw, be, bn, bs, bu, bd);
       const sys_table &t = p.tab[sys0 + m];
       const uint64_t f = FAST ? flip_fast3(t, g, c) : flip_general(t, 0, g, c, si, 7);
@@ -193,7 +221,8 @@ __global__ void __launch_bounds__(256) k_sweep3d_half(sweep_args p, int color) {
       if (z == 0) first[m] = sn;
       own[m] = down;
     }
-    jz_up = jd;
+    jz_up = cur.jd;
+    cur = nxt;
   }
 }
 

@@ -53,6 +53,7 @@ struct sa_ladder {
   uint64_t *words_d = nullptr, *words_h = nullptr;
   int64_t words_cap = 0;  // sweeps that fit in the staging buffers
   int *cnt_h = nullptr;
+  int sys_per_thread = 0;  // SA_TUNE_SYS_PER_THREAD, 0 = automatic
 };
 
 namespace {
@@ -106,9 +107,14 @@ void launch2d(const sa::sweep_args &a, int color, hipStream_t st) {
   hipLaunchKernelGGL((sa::k_sweep2d_half<COUNTER, FAST, OPEN, MB>), grid, dim3(256), 0, st, a, color);
 }
 
+int pick_mb(const sa_ladder *l) {
+  if (l->sys_per_thread > 0) return l->sys_per_thread;
+  return (l->nsys % 4 == 0) ? 4 : (l->nsys % 2 == 0) ? 2 : 1;
+}
+
 template <bool COUNTER, bool FAST>
 void launch_half(const sa_ladder *l, const sa::sweep_args &a, int color) {
-  const int mb = (l->nsys % 4 == 0) ? 4 : (l->nsys % 2 == 0) ? 2 : 1;
+  const int mb = pick_mb(l);
   if (l->kind == SA_KIND_3D) {
     if (mb == 4) launch3d<COUNTER, FAST, 4>(a, color, l->stream);
     else if (mb == 2) launch3d<COUNTER, FAST, 2>(a, color, l->stream);
@@ -439,6 +445,23 @@ int sa_ladder_swap(sa_ladder *l, int a, int b, uint64_t mask) {
   hipLaunchKernelGGL(sa::k_swap, dim3((unsigned)blocks), dim3(256), 0, l->stream, l->s + (int64_t)a * l->N,
                      l->s + (int64_t)b * l->N, mask, l->N);
   SA_HIP_TRY(hipGetLastError());
+  return SA_OK;
+}
+
+int sa_ladder_set_tuning(sa_ladder *l, int key, int value) {
+  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  if (key != SA_TUNE_SYS_PER_THREAD) return fail(SA_ERR_ARG, "unknown tuning key %d", key);
+  if (value != 0 && value != 1 && value != 2 && value != 4)
+    return fail(SA_ERR_ARG, "systems per thread must be 0 (auto), 1, 2 or 4");
+  if (value > 0 && l->nsys % value != 0) return fail(SA_ERR_ARG, "%d systems per thread do not divide nsys=%d", value, l->nsys);
+  l->sys_per_thread = value;
+  return SA_OK;
+}
+
+int sa_ladder_get_tuning(const sa_ladder *l, int key, int *value) {
+  if (!l || !value) return fail(SA_ERR_ARG, "null argument");
+  if (key != SA_TUNE_SYS_PER_THREAD) return fail(SA_ERR_ARG, "unknown tuning key %d", key);
+  *value = pick_mb(l);
   return SA_OK;
 }
 
