@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 KIND, L, NT, SWEEPS = 2, 128, 32, 10
 T_LADDER = [0.5 * 4.0 ** (k / (NT - 1)) for k in range(NT)]
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweep3d_L128_pt32_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweep3d_fused_L128_pt32_traffic.json")
 
 
 class HipEvents:
@@ -55,6 +55,10 @@ class HipEvents:
         ms = C.c_float()
         assert self.h.hipEventElapsedTime(C.byref(ms), a, b) == 0
         return ms.value
+
+
+def roofline_kernel(fused: bool) -> str:
+    return "k_sweep3d_fused" if fused else "k_sweep3d_half"
 
 
 def cpu_baseline():
@@ -125,10 +129,15 @@ def main():
     attempts = args.steps * SWEEPS * NT * N * 64
     value = attempts / dt / 1e9
 
-    # dominant kernel: the 3D half-sweep over the whole ladder.  Time back-to-back
-    # sweeps with hipEvents on the ladder's stream.
+    # dominant kernel: the 3D sweep over the whole ladder (one fused launch per
+    # sweep, or two half-sweep launches).  Time back-to-back sweeps with hipEvents
+    # on the ladder's stream.
     lad = pt.ladder_handle()
     stream = sa.lib().sa_ladder_stream(lad)
+    v = C.c_int()
+    sa.check(sa.lib().sa_ladder_get_tuning(lad, sa.TUNE_SWEEP3D_KERNEL, C.byref(v)), "tuning")
+    fused = v.value == 2
+    launches = 1 if fused else 2
     ev = HipEvents()
     a, b = ev.event(), ev.event()
     nsw = 40
@@ -136,19 +145,21 @@ def main():
     sa.check(sa.lib().sa_ladder_sweep(lad, nsw), "sweep")
     ev.record(b, stream)
     ms = ev.elapsed_ms(a, b)
-    launch_us = ms * 1e3 / (2 * nsw)
-    # algorithmic bytes per half-sweep launch: every spin word of the 32 systems read
-    # and written once per sweep (16 B per word-sweep) plus each coupling word once
-    # per sweep for the whole ladder (24 B per site, shared): half of it per launch.
-    alg_bytes = 0.5 * (NT * N * 16 + N * 24)
+    launch_us = ms * 1e3 / (launches * nsw)
+    # algorithmic bytes per sweep: every spin word of the 32 systems read and written
+    # once (16 B per word-sweep) plus each coupling word once for the whole ladder
+    # (24 B per site, shared by the 32 systems); divided over the sweep's launches.
+    alg_bytes = (NT * N * 16 + N * 24) / launches
     achieved = alg_bytes / (launch_us * 1e-6) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        if prof.get("kernel") == roofline_kernel(fused):
+            traffic = prof.get("hbm_bytes_per_launch")
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                    kernel="k_sweep3d_half", launch_us=round(launch_us, 2), alg_bytes_per_launch=alg_bytes)
+                    kernel=roofline_kernel(fused), launch_us=round(launch_us, 2), alg_bytes_per_launch=alg_bytes)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "replica spin-flip attempts/ns (1 GPU, 3D L=128)",

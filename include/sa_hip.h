@@ -118,10 +118,18 @@ int sa_ladder_measure(sa_ladder *l, int32_t *E, int32_t *M);
  * (swap, src/gpu_3d.cu:310-328, src/cpu_3d.cpp:429-444). */
 int sa_ladder_swap(sa_ladder *l, int a, int b, uint64_t mask);
 
-/* Kernel tuning (no effect on results).  SA_TUNE_SYS_PER_THREAD: systems one
- * thread updates per site (1, 2 or 4; it must divide nsys; 0 = automatic) -- more
- * share the coupling loads and the random word, fewer give more threads. */
-enum sa_tune_key { SA_TUNE_SYS_PER_THREAD = 1 };
+/* Kernel tuning (no effect on results; 0 = automatic everywhere).
+ *   SA_TUNE_SYS_PER_THREAD  systems one thread updates per site (1, 2 or 4, must
+ *                           divide nsys): more share coupling loads and random words
+ *   SA_TUNE_SWEEP3D_KERNEL  1 = two half-sweep launches, 2 = fused single pass
+ *   SA_TUNE_ROWS_PER_TILE   rows of the band a fused workgroup owns (4, 8, 16)
+ *   SA_TUNE_XCD_MAP         1 = keep a system's bands on one XCD (-1 automatic) */
+enum sa_tune_key {
+  SA_TUNE_SYS_PER_THREAD = 1,
+  SA_TUNE_SWEEP3D_KERNEL = 2,
+  SA_TUNE_ROWS_PER_TILE = 3,
+  SA_TUNE_XCD_MAP = 4
+};
 int sa_ladder_set_tuning(sa_ladder *l, int key, int value);
 int sa_ladder_get_tuning(const sa_ladder *l, int key, int *value);
 

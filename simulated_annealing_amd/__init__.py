@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(PKG, "libsa_hip.so")
 KIND_2P, KIND_2D, KIND_3D = 0, 1, 2
 RNG_REPLAY, RNG_COUNTER = 0, 1
 SA_OK = 0
-TUNE_SYS_PER_THREAD = 1
+TUNE_SYS_PER_THREAD, TUNE_SWEEP3D_KERNEL, TUNE_ROWS_PER_TILE, TUNE_XCD_MAP = 1, 2, 3, 4
 
 # every symbol include/sa_hip.h declares (checked by tests/test_capi_symbols.py)
 EXPORTS = [
@@ -225,11 +225,17 @@ class Ladder:
         check(lib().sa_ladder_sync(self.h), "sync")
 
     def set_sys_per_thread(self, mb: int):
-        check(lib().sa_ladder_set_tuning(self.h, TUNE_SYS_PER_THREAD, mb), "set_tuning")
+        self.tune(TUNE_SYS_PER_THREAD, mb)
 
     def sys_per_thread(self) -> int:
+        return self.tuning(TUNE_SYS_PER_THREAD)
+
+    def tune(self, key: int, value: int):
+        check(lib().sa_ladder_set_tuning(self.h, key, value), "set_tuning")
+
+    def tuning(self, key: int) -> int:
         v = C.c_int()
-        check(lib().sa_ladder_get_tuning(self.h, TUNE_SYS_PER_THREAD, C.byref(v)), "get_tuning")
+        check(lib().sa_ladder_get_tuning(self.h, key, C.byref(v)), "get_tuning")
         return v.value
 
 

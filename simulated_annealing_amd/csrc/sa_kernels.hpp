@@ -58,32 +58,51 @@ struct sweep_args {
 };
 
 // ---------------------------------------------------------------- bit-sliced logic
+// gfx950's v_bitop3_b32 evaluates any 3-input bitwise function in one VALU
+// instruction; the 8-bit truth table of f is f(0xF0, 0xCC, 0xAA) for operands
+// (a, b, c).  64-bit words are handled as two 32-bit halves.
+constexpr int B3_XOR3 = 0x96;  // a ^ b ^ c
+constexpr int B3_MAJ3 = 0xE8;  // majority(a, b, c)
+constexpr int B3_MUX = 0xCA;   // a ? b : c, per bit
+constexpr int B3_FLIP = 0x65;  // c ^ (~a | b): spin c flipped where ~a | b
+
+template <int IMM>
+__device__ __forceinline__ uint32_t b3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, IMM);
+}
+template <int IMM>
+__device__ __forceinline__ uint64_t b3(uint64_t a, uint64_t b, uint64_t c) {
+  const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, (uint32_t)c, IMM);
+  const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32), IMM);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 struct count3 {
   uint64_t b0, b1, b2;
 };
 
-__device__ __forceinline__ uint64_t maj3(uint64_t a, uint64_t b, uint64_t c) {
-  return (a & b) | (c & (a | b));
-}
+__device__ __forceinline__ uint64_t xor3(uint64_t a, uint64_t b, uint64_t c) { return b3<B3_XOR3>(a, b, c); }
+__device__ __forceinline__ uint64_t maj3(uint64_t a, uint64_t b, uint64_t c) { return b3<B3_MAJ3>(a, b, c); }
 
-// number of set bits among six words, per lane (0..6)
+// number of set bits among six words, per lane (0..6): two full adders and a
+// third on the carries
 __device__ __forceinline__ count3 count6(uint64_t a, uint64_t b, uint64_t c, uint64_t d, uint64_t e,
                                          uint64_t f) {
-  const uint64_t s1 = a ^ b ^ c, k1 = maj3(a, b, c);
-  const uint64_t s2 = d ^ e ^ f, k2 = maj3(d, e, f);
+  const uint64_t s1 = xor3(a, b, c), k1 = maj3(a, b, c);
+  const uint64_t s2 = xor3(d, e, f), k2 = maj3(d, e, f);
   const uint64_t k3 = s1 & s2;
-  return count3{s1 ^ s2, k1 ^ k2 ^ k3, maj3(k1, k2, k3)};
+  return count3{s1 ^ s2, xor3(k1, k2, k3), maj3(k1, k2, k3)};
 }
 
 // ... among four words (0..4)
 __device__ __forceinline__ count3 count4(uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
-  const uint64_t s1 = a ^ b ^ c, k1 = maj3(a, b, c);
+  const uint64_t s1 = xor3(a, b, c), k1 = maj3(a, b, c);
   const uint64_t k2 = s1 & d;
   return count3{s1 ^ d, k1 ^ k2, k1 & k2};
 }
 
 __device__ __forceinline__ uint64_t sel(uint64_t m, uint64_t a, uint64_t b) {  // m ? a : b, per bit
-  return (a & m) | (b & ~m);
+  return b3<B3_MUX>(m, a, b);
 }
 
 __device__ __forceinline__ uint64_t lvl(bool acc) { return acc ? ~0ULL : 0ULL; }
@@ -103,28 +122,65 @@ __device__ __forceinline__ uint64_t mux8(const count3 &c, const uint64_t a[8]) {
   return sel(c.b2, n1, n0);
 }
 
-// Flip mask of a site: general tables (any T, any h).
-__device__ __forceinline__ uint64_t flip_general(const sys_table &t, int tb0, uint64_t g, const count3 &c,
-                                                 uint64_t si, int nlev) {
+// New spin word of a site: general tables (any T, any h).
+__device__ __forceinline__ uint64_t update_general(const sys_table &t, int tb0, uint64_t g, const count3 &c,
+                                                   uint64_t si, int nlev) {
   uint64_t a[8];
   level_masks(t, tb0, g, a, nlev);
   const uint64_t f0 = mux8(c, a);
   level_masks(t, tb0 + 1, g, a, nlev);
   const uint64_t f1 = mux8(c, a);
-  return sel(si, f1, f0);
+  return si ^ sel(si, f1, f0);
 }
 
-// Flip mask when both spin tables are equal and every level c <= nb/2 always
-// accepts (h = 0, T > 0): only the levels above nb/2 need a comparison.
+// New spin word when both spin tables are equal and every level c <= nb/2 always
+// accepts (h = 0, T > 0): only the levels above nb/2 need a comparison.  The
+// accept decisions are per site, so they are built once as 32-bit masks and
+// applied to both halves of the word.
 //   3D (levels 4,5,6 thresholded): flip = ~c2 | (c1 ? a6 : (c0 ? a5 : a4))
 //   2D (levels 3,4):               flip = c2 ? a4 : ((c1 & c0) ? a3 : ~0)
-__device__ __forceinline__ uint64_t flip_fast3(const sys_table &t, uint64_t g, const count3 &c) {
-  const uint64_t a4 = lvl(g < t.thr[0][4]), a5 = lvl(g < t.thr[0][5]), a6 = lvl(g < t.thr[0][6]);
-  return ~c.b2 | sel(c.b1, a6, sel(c.b0, a5, a4));
+__device__ __forceinline__ uint64_t update_fast3(const sys_table &t, uint64_t g, const count3 &c, uint64_t si) {
+  const uint32_t a4 = (g < t.thr[0][4]) ? ~0u : 0u, a5 = (g < t.thr[0][5]) ? ~0u : 0u;
+  const uint32_t a6 = (g < t.thr[0][6]) ? ~0u : 0u;
+  const uint32_t vlo = b3<B3_MUX>((uint32_t)c.b1, a6, b3<B3_MUX>((uint32_t)c.b0, a5, a4));
+  const uint32_t vhi = b3<B3_MUX>((uint32_t)(c.b1 >> 32), a6, b3<B3_MUX>((uint32_t)(c.b0 >> 32), a5, a4));
+  const uint32_t lo = b3<B3_FLIP>((uint32_t)c.b2, vlo, (uint32_t)si);
+  const uint32_t hi = b3<B3_FLIP>((uint32_t)(c.b2 >> 32), vhi, (uint32_t)(si >> 32));
+  return ((uint64_t)hi << 32) | lo;
 }
-__device__ __forceinline__ uint64_t flip_fast2(const sys_table &t, uint64_t g, const count3 &c) {
+__device__ __forceinline__ uint64_t update_fast2(const sys_table &t, uint64_t g, const count3 &c, uint64_t si) {
   const uint64_t a3 = lvl(g < t.thr[0][3]), a4 = lvl(g < t.thr[0][4]);
-  return sel(c.b2, a4, ~(c.b1 & c.b0) | a3);
+  return si ^ sel(c.b2, a4, ~(c.b1 & c.b0) | a3);
+}
+
+// ---------------------------------------------------------------- buffer access
+// Raw buffer loads/stores: a 128-bit resource (base, size) in scalar registers, a
+// per-thread 32-bit byte offset that stays fixed for the whole z-walk, and the
+// plane offset as a wave-uniform scalar -- no per-load 64-bit address arithmetic.
+typedef unsigned int sa_v2u32 __attribute__((ext_vector_type(2)));
+typedef unsigned int sa_v4u32 __attribute__((ext_vector_type(4)));
+typedef __amdgpu_buffer_rsrc_t buf_t;
+
+__device__ __forceinline__ buf_t make_buf(const void *p, uint32_t bytes) {
+  // dword 3 = 0x00020000: raw (untyped) dword access on gfx9-family GPUs
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint64_t bld(buf_t b, uint32_t voff, uint32_t soff) {
+  const sa_v2u32 v = __builtin_amdgcn_raw_buffer_load_b64(b, (int)voff, (int)soff, 0);
+  return ((uint64_t)v.y << 32) | v.x;
+}
+__device__ __forceinline__ void bld2(buf_t b, uint32_t voff, uint32_t soff, uint64_t &a, uint64_t &c) {
+  const sa_v4u32 v = __builtin_amdgcn_raw_buffer_load_b128(b, (int)voff, (int)soff, 0);
+  a = ((uint64_t)v.y << 32) | v.x;
+  c = ((uint64_t)v.w << 32) | v.z;
+}
+__device__ __forceinline__ void bst2(buf_t b, uint32_t voff, uint32_t soff, uint64_t a, uint64_t c) {
+  sa_v4u32 v;
+  v.x = (uint32_t)a;
+  v.y = (uint32_t)(a >> 32);
+  v.z = (uint32_t)c;
+  v.w = (uint32_t)(c >> 32);
+  __builtin_amdgcn_raw_buffer_store_b128(v, b, (int)voff, (int)soff, 0);
 }
 
 // ---------------------------------------------------------------- 3D half-sweep
@@ -206,16 +262,10 @@ __global__ void __launch_bounds__(256) k_sweep3d_half(sweep_args p, int color) {
     for (int m = 0; m < MB; ++m) {
       const uint64_t si = own[m];
       const uint64_t down = (z == L - 1) ? first[m] : cur.d[m];
-      const uint64_t bw = cur.jw ^ si ^ cur.w[m];
-      const uint64_t be = cur.je ^ si ^ cur.e[m];
-      const uint64_t bn = cur.jn ^ si ^ cur.n[m];
-      const uint64_t bs = cur.js ^ si ^ cur.s[m];
-      const uint64_t bu = jz_up ^ si ^ up[m];
-      const uint64_t bd = cur.jd ^ si ^ down;
-      const count3 c = count6(bw, be, bn, bs, bu, bd);
+      const count3 c = count6(xor3(cur.jw, si, cur.w[m]), xor3(cur.je, si, cur.e[m]), xor3(cur.jn, si, cur.n[m]),
+                              xor3(cur.js, si, cur.s[m]), xor3(jz_up, si, up[m]), xor3(cur.jd, si, down));
       const sys_table &t = p.tab[sys0 + m];
-      const uint64_t f = FAST ? flip_fast3(t, g, c) : flip_general(t, 0, g, c, si, 7);
-      const uint64_t sn = si ^ f;
+      const uint64_t sn = FAST ? update_fast3(t, g, c, si) : update_general(t, 0, g, c, si, 7);
       sp[m][P + c0] = sn;
       up[m] = sn;
       if (z == 0) first[m] = sn;
@@ -223,6 +273,264 @@ __global__ void __launch_bounds__(256) k_sweep3d_half(sweep_args p, int color) {
     }
     jz_up = cur.jd;
     cur = nxt;
+  }
+}
+
+// ---------------------------------------------------------------- 3D fused sweep
+// One launch per sweep.  A workgroup owns a band of TH full rows (y0 .. y0+TH-1)
+// of MB systems and walks z once; at every plane it performs the offset-0 half
+// (colour 0) and then the offset-1 half (colour 1):
+//   * colour-0 site at plane z needs old colour-1 in-plane neighbours, the new
+//     colour-0 word at z-1 and the old one at z+1 -- exactly what the reference's
+//     first half-sweep sees there, since that half never touches colour 1 and walks
+//     z upwards (src/cpu_3d.cpp:94-105);
+//   * colour-1 site at plane z needs the NEW colour-0 in-plane neighbours of the
+//     same plane, the new colour-1 word at z-1 and the old one at z+1.
+// The colour-0 updates of the rows just outside the band (y0-1, y0+TH) are
+// recomputed by the workgroup (identical to the neighbour's), so no workgroup
+// waits for another.  Spins are read from one array and written to another
+// (ping-pong per sweep), so every "old" value a workgroup reads is untouched by
+// any other workgroup during the sweep.
+//
+// Threads: (TH+2) rows x L/2 site pairs (2k, 2k+1); row 0 and row TH+1 are the
+// halo rows.  LDS holds, per plane buffer, row and system, the colour-0 and the
+// colour-1 word of every pair: c0[k], c1[k].  In a row of parity P the colour-0
+// site of pair k is x0 = 2k+P and the colour-1 site x1 = 2k+1-P, so
+//   colour-0 x-neighbours: c1[k], c1[k + (P ? 1 : -1)]   y-neighbours: c1[k] of rows r-1, r+1
+//   colour-1 x-neighbours: c0[k], c0[k + (P ? -1 : 1)]   y-neighbours: c0[k] of rows r-1, r+1
+// Two barriers per plane; the plane buffers alternate so the next plane's writes
+// never meet the current plane's reads.  A thread's row parity is fixed, so the
+// z-walk is instantiated per parity when a row spans whole waves (L/2 a multiple
+// of 64), where the choice is wave-uniform; smaller lattices keep the parity as a
+// run-time value.  z advances two planes per
+// iteration (one Philox block serves a column's pair of planes) with two load
+// buffers used alternately.
+template <int MB>
+struct fused_loads {
+  uint64_t na[MB], nb[MB];  // own pair at plane z+1 (old)
+  uint64_t ring2[MB];       // halo rows: colour-1 word one row further out (old)
+  uint64_t jxa, jxb, jxn;   // jx of 2k, 2k+1, 2k+2
+  uint64_t jya, jyb;        // jy of the pair (bond to y-1)
+  uint64_t jsa, jsb;        // jy of the pair in row y+1 (bond to y+1)
+  uint64_t jda, jdb;        // jz of the pair at plane z+1 (bond to z+1)
+  uint64_t g0, g1;          // replay words of the colour-0 / colour-1 site
+};
+
+template <bool COUNTER, bool FAST, int MB, int TH>
+struct fused_ctx {
+  const sweep_args &p;
+  uint64_t *lds;
+  int L, PR, r, k, kn, kp, y, row, rowS, yr2, sys0;
+  bool halo;
+  int64_t L2, N, rank0;
+  int par;  // row parity (the template argument P is -1 when it is only known at run time)
+  buf_t sin[MB], sout[MB], bjx, bjy, bjz, bw;
+  uint32_t vpair, vpairS, vnext, vring, vrank;  // per-thread byte offsets
+  uint32_t plane_bytes;                          // L*L*8
+
+  __device__ __forceinline__ uint64_t &c0(int buf, int rr, int m, int kk) const {
+    return lds[(((buf * (TH + 2) + rr) * MB + m) * 2 + 0) * PR + kk];
+  }
+  __device__ __forceinline__ uint64_t &c1(int buf, int rr, int m, int kk) const {
+    return lds[(((buf * (TH + 2) + rr) * MB + m) * 2 + 1) * PR + kk];
+  }
+
+  __device__ __forceinline__ void load(int z, fused_loads<MB> &q) const {
+    const uint32_t P = (uint32_t)z * plane_bytes;
+    const uint32_t Pd = (z == L - 1) ? 0u : P + plane_bytes;
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      if (z == L - 1) {
+        q.na[m] = q.nb[m] = 0;
+      } else {
+        bld2(sin[m], vpair, Pd, q.na[m], q.nb[m]);
+      }
+      q.ring2[m] = halo ? bld(sin[m], vring, P) : 0;
+    }
+    bld2(bjx, vpair, P, q.jxa, q.jxb);
+    q.jxn = bld(bjx, vnext, P);
+    bld2(bjy, vpair, P, q.jya, q.jyb);
+    bld2(bjy, vpairS, P, q.jsa, q.jsb);
+    bld2(bjz, vpair, Pd, q.jda, q.jdb);
+    if (!COUNTER) {
+      const uint32_t half_plane = (uint32_t)z * (plane_bytes >> 1);
+      q.g0 = bld(bw, vrank, half_plane);
+      q.g1 = halo ? 0 : bld(bw, vrank, half_plane + (uint32_t)(N >> 1) * 8u);
+    }
+  }
+
+  // one plane: both halves.  pa/pb: the pair's old words at plane z (in),
+  // replaced by the old words at plane z+1 (out).
+  template <int PT>
+  __device__ __forceinline__ void plane(int z, const fused_loads<MB> &cur, uint64_t g0, uint64_t g1, uint64_t *pa,
+                                        uint64_t *pb, uint64_t *up0, uint64_t *up1, uint64_t *first0,
+                                        uint64_t *first1, uint64_t &ju0, uint64_t &ju1) const {
+    const int P = PT >= 0 ? PT : par;
+    const int buf = z & 1;
+    // colour-0 word = element P of the pair, colour-1 word = element 1-P
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      c0(buf, r, m, k) = P ? pb[m] : pa[m];
+      c1(buf, r, m, k) = P ? pa[m] : pb[m];
+    }
+    const uint64_t j0w = P ? cur.jxb : cur.jxa, j0e = P ? cur.jxn : cur.jxb;
+    const uint64_t j1w = P ? cur.jxa : cur.jxb, j1e = P ? cur.jxb : cur.jxn;
+    const uint64_t j0n = P ? cur.jyb : cur.jya, j0s = P ? cur.jsb : cur.jsa;
+    const uint64_t j1n = P ? cur.jya : cur.jyb, j1s = P ? cur.jsa : cur.jsb;
+    const uint64_t j0d = P ? cur.jdb : cur.jda, j1d = P ? cur.jda : cur.jdb;
+    const bool last = z == L - 1;
+    __syncthreads();
+    // ---- offset-0 half: colour-0 site of every row, halo rows included
+    uint64_t n0[MB];
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      const uint64_t si = P ? pb[m] : pa[m];
+      const uint64_t w = c1(buf, r, m, P ? k : kp);
+      const uint64_t e = c1(buf, r, m, P ? kn : k);
+      const uint64_t nn = (r == 0) ? cur.ring2[m] : c1(buf, r - 1, m, k);
+      const uint64_t ss = (r == TH + 1) ? cur.ring2[m] : c1(buf, r + 1, m, k);
+      const uint64_t down = last ? first0[m] : (P ? cur.nb[m] : cur.na[m]);
+      const count3 c = count6(xor3(j0w, si, w), xor3(j0e, si, e), xor3(j0n, si, nn), xor3(j0s, si, ss),
+                              xor3(ju0, si, up0[m]), xor3(j0d, si, down));
+      const sys_table &t = p.tab[sys0 + m];
+      n0[m] = FAST ? update_fast3(t, g0, c, si) : update_general(t, 0, g0, c, si, 7);
+      c0(buf, r, m, k) = n0[m];
+      up0[m] = n0[m];
+      if (z == 0) first0[m] = n0[m];
+    }
+    __syncthreads();
+    // ---- offset-1 half: colour-1 site of the band rows
+    if (!halo) {
+#pragma unroll
+      for (int m = 0; m < MB; ++m) {
+        const uint64_t si = P ? pa[m] : pb[m];
+        const uint64_t w = c0(buf, r, m, P ? kp : k);
+        const uint64_t e = c0(buf, r, m, P ? k : kn);
+        const uint64_t nn = c0(buf, r - 1, m, k);
+        const uint64_t ss = c0(buf, r + 1, m, k);
+        const uint64_t down = last ? first1[m] : (P ? cur.na[m] : cur.nb[m]);
+        const count3 c = count6(xor3(j1w, si, w), xor3(j1e, si, e), xor3(j1n, si, nn), xor3(j1s, si, ss),
+                                xor3(ju1, si, up1[m]), xor3(j1d, si, down));
+        const sys_table &t = p.tab[sys0 + m];
+        const uint64_t n1 = FAST ? update_fast3(t, g1, c, si) : update_general(t, 0, g1, c, si, 7);
+        up1[m] = n1;
+        if (z == 0) first1[m] = n1;
+        bst2(sout[m], vpair, (uint32_t)z * plane_bytes, P ? n1 : n0[m], P ? n0[m] : n1);
+      }
+    }
+    ju0 = j0d;
+    ju1 = j1d;
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      pa[m] = cur.na[m];
+      pb[m] = cur.nb[m];
+    }
+  }
+
+  template <int PT>
+  __device__ __forceinline__ void walk() const {
+    const int P = PT >= 0 ? PT : par;
+    const int x0 = 2 * k + P, x1 = 2 * k + 1 - P;
+    uint64_t pa[MB], pb[MB], up0[MB], up1[MB], first0[MB], first1[MB];
+    const uint32_t top = (uint32_t)(L - 1) * plane_bytes;
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      bld2(sin[m], vpair, 0u, pa[m], pb[m]);
+      uint64_t ta, tb;
+      bld2(sin[m], vpair, top, ta, tb);  // old plane L-1
+      up0[m] = P ? tb : ta;
+      up1[m] = P ? ta : tb;
+      first0[m] = first1[m] = 0;
+    }
+    uint64_t ja, jb;
+    bld2(bjz, vpair, 0u, ja, jb);  // -z bonds at plane 0
+    uint64_t ju0 = P ? jb : ja, ju1 = P ? ja : jb;
+    fused_loads<MB> A, B;
+    load(0, A);
+    for (int z = 0; z < L; z += 2) {  // L is even
+      load(z + 1, B);
+      uint64_t g0a, g1a, g0b, g1b;
+      if (COUNTER) {
+        const u32x4 blk0 = counter_block((uint32_t)(x0 + row + z * L2), p.seed, p.sweep);
+        const u32x4 blk1 = halo ? blk0 : counter_block((uint32_t)(x1 + row + z * L2), p.seed, p.sweep);
+        g0a = counter_half(blk0, 0);
+        g0b = counter_half(blk0, 1);
+        g1a = counter_half(blk1, 0);
+        g1b = counter_half(blk1, 1);
+      } else {
+        g0a = A.g0;
+        g1a = A.g1;
+        g0b = B.g0;
+        g1b = B.g1;
+      }
+      plane<PT>(z, A, g0a, g1a, pa, pb, up0, up1, first0, first1, ju0, ju1);
+      if (z + 2 < L) load(z + 2, A);
+      plane<PT>(z + 1, B, g0b, g1b, pa, pb, up0, up1, first0, first1, ju0, ju1);
+    }
+  }
+};
+
+template <bool COUNTER, bool FAST, int MB, int TH>
+__global__ void __launch_bounds__(640) k_sweep3d_fused(sweep_args p, uint64_t *__restrict__ s_out, int xcd_map) {
+  extern __shared__ uint64_t lds[];
+  const int L = p.L, PR = L >> 1;
+  const int ntile = L / TH;
+  // workgroup -> (system block, band).  Workgroups are dispatched round-robin over
+  // the 8 XCDs (L2 is per XCD).  xcd_map = 0: b = sb*ntile + band, so with ntile a
+  // multiple of 8 every system block of a band runs on one XCD and shares its
+  // couplings in L2.  xcd_map = 1: XCD b%8 gets ntile/8 adjacent bands of every
+  // system block, so the halo rows between those bands are L2 hits too.
+  const int b = blockIdx.x;
+  int sb, tile;
+  if (xcd_map) {
+    const int per = ntile >> 3, xcd = b & 7, q = b >> 3;
+    tile = xcd * per + q % per;
+    sb = q / per;
+  } else {
+    sb = b / ntile;
+    tile = b - sb * ntile;
+  }
+  fused_ctx<COUNTER, FAST, MB, TH> c{p, lds};
+  c.L = L;
+  c.PR = PR;
+  c.r = threadIdx.x / PR;
+  c.k = threadIdx.x - c.r * PR;
+  c.kn = c.k == PR - 1 ? 0 : c.k + 1;
+  c.kp = c.k == 0 ? PR - 1 : c.k - 1;
+  c.halo = (c.r == 0) || (c.r == TH + 1);
+  c.y = (tile * TH - 1 + c.r + L) % L;
+  const int ym = c.y == 0 ? L - 1 : c.y - 1, yp = c.y == L - 1 ? 0 : c.y + 1;
+  c.row = c.y * L;
+  c.rowS = yp * L;
+  c.yr2 = (c.r == 0) ? ym : yp;  // halo rows: the row beyond the halo
+  c.sys0 = sb * MB;
+  c.L2 = (int64_t)L * L;
+  c.N = p.N;
+  c.rank0 = (int64_t)c.y * PR + c.k;  // visit rank inside a plane half
+  const uint32_t bytes = (uint32_t)(c.N * 8);
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {
+    c.sin[m] = make_buf(p.s + (int64_t)(c.sys0 + m) * c.N, bytes);
+    c.sout[m] = make_buf(s_out + (int64_t)(c.sys0 + m) * c.N, bytes);
+  }
+  c.bjx = make_buf(p.jx, bytes);
+  c.bjy = make_buf(p.jy, bytes);
+  c.bjz = make_buf(p.jz, bytes);
+  c.bw = make_buf(p.words, COUNTER ? 0u : bytes);
+  c.plane_bytes = (uint32_t)(c.L2 * 8);
+  c.vpair = (uint32_t)(c.row + 2 * c.k) * 8u;
+  c.vpairS = (uint32_t)(c.rowS + 2 * c.k) * 8u;
+  c.vnext = (uint32_t)(c.row + 2 * c.kn) * 8u;
+  c.vring = (uint32_t)(c.yr2 * L + 2 * c.k + (c.y & 1)) * 8u;
+  c.vrank = (uint32_t)c.rank0 * 8u;
+  c.par = c.y & 1;
+  // The walk contains barriers, so a parity-specialised walk may only be chosen
+  // per wave: when a row is a whole number of waves (L/2 a multiple of 64) every
+  // wave has one parity; otherwise the parity stays a run-time value.
+  if ((PR & 63) == 0) {
+    if (c.par) c.template walk<1>(); else c.template walk<0>();
+  } else {
+    c.template walk<-1>();
   }
 }
 
@@ -253,77 +561,164 @@ __global__ void __launch_bounds__(256) k_sweep2d_half(sweep_args p, int color) {
   for (int m = 0; m < MB; ++m) {
     uint64_t *s = p.s + (int64_t)(sys0 + m) * p.N;
     const uint64_t si = s[c0];
-    uint64_t bw = jw ^ si ^ s[cw];
-    uint64_t be = je ^ si ^ s[ce];
+    uint64_t bw = xor3(jw, si, s[cw]);
+    uint64_t be = xor3(je, si, s[ce]);
     if (OPEN) {  // missing open-x bond counts as satisfied (src/cpu_2dp.cpp:338-339)
       if (x == 0) bw = ~0ULL;
       if (x == L - 1) be = ~0ULL;
     }
-    const uint64_t bn = jn ^ si ^ s[cn];
-    const uint64_t bs = js ^ si ^ s[cs];
+    const uint64_t bn = xor3(jn, si, s[cn]);
+    const uint64_t bs = xor3(js, si, s[cs]);
     const count3 c = count4(bw, be, bn, bs);
     const sys_table &t = p.tab[sys0 + m];
-    uint64_t f;
-    if (FAST && !OPEN) f = flip_fast2(t, g, c);
-    else f = flip_general(t, border ? 2 : 0, g, c, si, 5);
-    s[c0] = si ^ f;
+    s[c0] = (FAST && !OPEN) ? update_fast2(t, g, c, si) : update_general(t, border ? 2 : 0, g, c, si, 5);
   }
 }
 
 // ---------------------------------------------------------------- measure
 // Per system and lane: number of satisfied -x,-y[,-z] bonds and of set spins.
-// A wave stages 64 site words (and their bond words) in LDS; lane k then walks
-// the 64 staged words and counts bit k.  Counts are reduced in LDS per block and
-// added to cnt[sys][0|1][lane] with one atomic per lane per block.
+// Each lane keeps bit-sliced counters (bit k of plane b = bit b of lane k's count):
+// a full adder folds the bond words of one site into a 2-bit sum that is rippled
+// into six planes; the spin word ripples into five.  Every 16 site groups the
+// planes are turned into ordinary per-lane counts by a 64x64 bit transpose across
+// the wave (six butterfly stages of lane shuffles) and a popcount, so lane k ends
+// up holding the counts of bit k.  Blocks reduce in LDS and add one atomic per
+// lane and counter.
+
+// Wave-wide 64x64 bit transpose: lane i holds row i on entry, column i on exit.
+// Stage d exchanges bit d of the lane index with bit d of the bit index.
+__device__ __forceinline__ uint64_t wave_transpose64(uint64_t x, int lane) {
+  constexpr uint64_t M32 = 0x00000000FFFFFFFFULL, M16 = 0x0000FFFF0000FFFFULL, M8 = 0x00FF00FF00FF00FFULL,
+                     M4 = 0x0F0F0F0F0F0F0F0FULL, M2 = 0x3333333333333333ULL, M1 = 0x5555555555555555ULL;
+#define SA_TSTAGE(D, M)                                                            \
+  {                                                                                \
+    const uint64_t o = __shfl_xor(x, D);                                           \
+    x = (lane & D) ? ((x & ~M) | ((o & ~M) >> D)) : ((x & M) | ((o & M) << D));    \
+  }
+  SA_TSTAGE(32, M32)
+  SA_TSTAGE(16, M16)
+  SA_TSTAGE(8, M8)
+  SA_TSTAGE(4, M4)
+  SA_TSTAGE(2, M2)
+  SA_TSTAGE(1, M1)
+#undef SA_TSTAGE
+  return x;
+}
+
 template <int KIND>
 __global__ void __launch_bounds__(256) k_measure(const uint64_t *__restrict__ s_all, const uint64_t *__restrict__ jx,
                                                  const uint64_t *__restrict__ jy, const uint64_t *__restrict__ jz,
-                                                 int L, int64_t N, int *cnt) {
-  constexpr int NB = (KIND == KIND_3D) ? 3 : 2;
-  __shared__ uint64_t stage[4][NB + 1][64];
-  __shared__ int red[2][64];
-  const int sys = blockIdx.y;
-  const uint64_t *s = s_all + (int64_t)sys * N;
+                                                 int L, int64_t N, int nsys, int *cnt) {
+  constexpr int FLUSH = 16;  // groups between flushes: E <= 3*16 < 64, M <= 16 < 32
+  constexpr int SB = 4;      // systems per pass over the sites (couplings loaded once per pass)
+  __shared__ int red[SB][2][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (threadIdx.x < 128) red[threadIdx.x >> 6][threadIdx.x & 63] = 0;
-  int sat = 0, up = 0;
+  const int sys0 = blockIdx.y * SB;
+  const int ns = (nsys - sys0) < SB ? (nsys - sys0) : SB;
+  for (int q = threadIdx.x; q < SB * 128; q += blockDim.x) (&red[0][0][0])[q] = 0;
+  __syncthreads();
   const int64_t L2 = (int64_t)L * L;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  uint64_t e[SB][6], mc[SB][5];
+  int sat[SB], up[SB];
+#pragma unroll
+  for (int m = 0; m < SB; ++m) {
+    sat[m] = up[m] = 0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) e[m][b] = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) mc[m][b] = 0;
+  }
+  int pending = 0;
+  auto flush = [&]() {
+#pragma unroll
+    for (int m = 0; m < SB; ++m) {
+#pragma unroll
+      for (int b = 0; b < 6; ++b) {
+        sat[m] += __popcll(wave_transpose64(e[m][b], lane)) << b;
+        e[m][b] = 0;
+      }
+#pragma unroll
+      for (int b = 0; b < 5; ++b) {
+        up[m] += __popcll(wave_transpose64(mc[m][b], lane)) << b;
+        mc[m][b] = 0;
+      }
+    }
+  };
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x + wv * 64; base < N; base += stride) {
-    const int64_t i = base + lane;
-    uint64_t w = 0, bw = 0, bn = 0, bu = 0;
-    if (i < N) {
-      const int x = (int)(i % L);
-      const int64_t y = (i / L) % L;
-      const int64_t z = i / L2;
-      w = s[i];
-      const int64_t iw = i - x + (x == 0 ? L - 1 : x - 1);
-      const int64_t in = i + ((y == 0 ? L - 1 : y - 1) - y) * L;
-      bw = jx[i] ^ w ^ s[iw];
-      if (KIND == KIND_2P && x == 0) bw = 0;  // no open-x west bond
-      bn = jy[i] ^ w ^ s[in];
-      if (KIND == KIND_3D) bu = jz[i] ^ w ^ s[i + ((z == 0 ? L - 1 : z - 1) - z) * L2];
+    // coordinates of the chunk's first site once per wave (scalar), then per lane
+    // by stepping along the row: no per-site division
+    const uint32_t b32 = (uint32_t)base, uL = (uint32_t)L, uL2 = (uint32_t)L2;
+    const uint32_t zb = b32 / uL2, rem = b32 - zb * uL2, yb = rem / uL, xb = rem - yb * uL;
+    uint32_t x = xb + (uint32_t)lane, y = yb, z = zb;
+    while (x >= uL) {  // at most 64/L steps
+      x -= uL;
+      if (++y == uL) {
+        y = 0;
+        ++z;
+      }
     }
-    stage[wv][0][lane] = w;
-    stage[wv][1][lane] = bw;
-    stage[wv][2][lane] = bn;
-    if (KIND == KIND_3D) stage[wv][3][lane] = bu;
-    __builtin_amdgcn_wave_barrier();
-    const int n = (int)((N - base) < 64 ? (N - base) : 64);
-    for (int r = 0; r < n; ++r) {
-      up += (int)((stage[wv][0][r] >> lane) & 1ULL);
-      int b = (int)((stage[wv][1][r] >> lane) & 1ULL) + (int)((stage[wv][2][r] >> lane) & 1ULL);
-      if (KIND == KIND_3D) b += (int)((stage[wv][3][r] >> lane) & 1ULL);
-      sat += b;
+    const uint32_t i = b32 + (uint32_t)lane;
+    const bool in = (int64_t)i < N;
+    uint32_t iw = 0, inn = 0, iu = 0;
+    uint64_t Jw = 0, Jn = 0, Ju = 0;
+    bool west = true;
+    if (in) {
+      iw = i - x + (x == 0 ? uL - 1 : x - 1);
+      inn = (y == 0) ? i + uL2 - uL : i - uL;
+      Jw = jx[i];
+      Jn = jy[i];
+      west = !(KIND == KIND_2P && x == 0);  // no open-x west bond
+      if (KIND == KIND_3D) {
+        iu = (z == 0) ? i + (uint32_t)(N - L2) : i - uL2;
+        Ju = jz[i];
+      }
     }
-    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int m = 0; m < SB; ++m) {
+      if (m >= ns) break;
+      const uint64_t *s = s_all + (int64_t)(sys0 + m) * N;
+      uint64_t w = 0, bw = 0, bn = 0, bu = 0;
+      if (in) {
+        w = s[i];
+        bw = west ? (Jw ^ w ^ s[iw]) : 0;
+        bn = Jn ^ w ^ s[inn];
+        if (KIND == KIND_3D) bu = Ju ^ w ^ s[iu];
+      }
+      // satisfied bonds of this site: sum bit sb (weight 1) and carry cb (weight 2)
+      const uint64_t sb = bw ^ bn ^ bu, cb = maj3(bw, bn, bu);
+      uint64_t t = e[m][0] & sb;
+      e[m][0] ^= sb;
+      uint64_t u = maj3(e[m][1], cb, t);
+      e[m][1] ^= cb ^ t;
+#pragma unroll
+      for (int b = 2; b < 6; ++b) {
+        const uint64_t v = e[m][b] & u;
+        e[m][b] ^= u;
+        u = v;
+      }
+      t = w;
+#pragma unroll
+      for (int b = 0; b < 5; ++b) {
+        const uint64_t v = mc[m][b] & t;
+        mc[m][b] ^= t;
+        t = v;
+      }
+    }
+    if (++pending == FLUSH) {
+      flush();
+      pending = 0;
+    }
+  }
+  if (pending) flush();
+#pragma unroll
+  for (int m = 0; m < SB; ++m) {
+    atomicAdd(&red[m][0][lane], sat[m]);
+    atomicAdd(&red[m][1][lane], up[m]);
   }
   __syncthreads();
-  atomicAdd(&red[0][lane], sat);
-  atomicAdd(&red[1][lane], up);
-  __syncthreads();
-  if (threadIdx.x < 128) atomicAdd(&cnt[(sys * 2 + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63)],
-                                   red[threadIdx.x >> 6][threadIdx.x & 63]);
+  for (int q = threadIdx.x; q < ns * 128; q += blockDim.x)
+    atomicAdd(&cnt[(int64_t)sys0 * 128 + q], (&red[0][0][0])[q]);
 }
 
 __global__ void k_fill_zero(int *p, int n) {

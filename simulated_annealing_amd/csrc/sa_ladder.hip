@@ -43,6 +43,7 @@ struct sa_ladder {
   int device = 0;
   hipStream_t stream = nullptr;
   uint64_t *s = nullptr, *jx = nullptr, *jy = nullptr, *jz = nullptr;
+  uint64_t *s_alt = nullptr;  // second spin array of the fused 3D sweep (ping-pong)
   sa::sys_table *tab = nullptr;
   int *cnt = nullptr;
   std::vector<sa::sys_table> htab;
@@ -54,6 +55,9 @@ struct sa_ladder {
   int64_t words_cap = 0;  // sweeps that fit in the staging buffers
   int *cnt_h = nullptr;
   int sys_per_thread = 0;  // SA_TUNE_SYS_PER_THREAD, 0 = automatic
+  int sweep3d_kernel = 0;  // SA_TUNE_SWEEP3D_KERNEL, 0 = automatic
+  int rows_per_tile = 0;   // SA_TUNE_ROWS_PER_TILE, 0 = automatic
+  int xcd_map = -1;        // SA_TUNE_XCD_MAP, -1 = automatic
 };
 
 namespace {
@@ -67,6 +71,7 @@ void free_all(sa_ladder *l) {
   if (l->jz) (void)hipFree(l->jz);
   if (l->tab) (void)hipFree(l->tab);
   if (l->cnt) (void)hipFree(l->cnt);
+  if (l->s_alt) (void)hipFree(l->s_alt);
   if (l->words_d) (void)hipFree(l->words_d);
   if (l->words_h) (void)hipHostFree(l->words_h);
   if (l->cnt_h) (void)hipHostFree(l->cnt_h);
@@ -130,7 +135,72 @@ void launch_half(const sa_ladder *l, const sa::sweep_args &a, int color) {
   }
 }
 
-void launch_sweep(const sa_ladder *l, const uint64_t *words, uint64_t t) {
+// ---- fused 3D sweep: configuration (rows per tile TH, systems per workgroup MB)
+struct fused_cfg {
+  int th, mb, xcd;
+};
+
+size_t fused_lds_bytes(int L, int th, int mb) { return (size_t)2 * (th + 2) * mb * 2 * (L / 2) * 8; }
+
+bool fused_ok(const sa_ladder *l, int th, int mb) {
+  if (l->kind != SA_KIND_3D || l->L % th != 0 || th > l->L) return false;
+  if (l->nsys % mb != 0) return false;
+  if ((th + 2) * (l->L / 2) > 640) return false;  // the kernel's launch bound
+  return fused_lds_bytes(l->L, th, mb) <= 64 * 1024;
+}
+
+// automatic choice: the largest band that fits (less halo recomputation), two
+// systems per workgroup when the ladder has enough of them to fill the GPU
+fused_cfg pick_fused(const sa_ladder *l) {
+  fused_cfg c{0, 1, 0};
+  const int mbs[2] = {l->sys_per_thread > 0 ? l->sys_per_thread : 2, 1};
+  const int ths[3] = {16, 8, 4};
+  for (int mb : mbs) {
+    if (mb != 1 && mb != 2) continue;
+    for (int th : ths) {
+      if (l->rows_per_tile > 0 && th != l->rows_per_tile) continue;
+      if (!fused_ok(l, th, mb)) continue;
+      // two systems per workgroup share couplings and random words, unless that
+      // leaves fewer workgroups than CUs
+      if (mb == 2 && l->sys_per_thread != 2 && (l->L / th) * (l->nsys / mb) < 256) continue;
+      c.th = th;
+      c.mb = mb;
+      const int ntile = l->L / th;
+      const bool can = ntile % 8 == 0;
+      c.xcd = (l->xcd_map < 0) ? 0 : (l->xcd_map && can);
+      return c;
+    }
+  }
+  return c;
+}
+
+bool use_fused(const sa_ladder *l) {
+  if (l->kind != SA_KIND_3D || l->sweep3d_kernel == 1) return false;
+  return pick_fused(l).th > 0;
+}
+
+template <bool COUNTER, bool FAST, int MB, int TH>
+void launch_fused_t(const sa::sweep_args &a, uint64_t *out, const fused_cfg &c, hipStream_t st) {
+  const int G = (a.L / TH) * (a.nsys / MB);
+  const int threads = (TH + 2) * (a.L / 2);
+  hipLaunchKernelGGL((sa::k_sweep3d_fused<COUNTER, FAST, MB, TH>), dim3(G), dim3(threads),
+                     fused_lds_bytes(a.L, TH, MB), st, a, out, c.xcd);
+}
+
+template <bool COUNTER, bool FAST>
+void launch_fused(const sa::sweep_args &a, uint64_t *out, const fused_cfg &c, hipStream_t st) {
+  if (c.mb == 2) {
+    if (c.th == 16) launch_fused_t<COUNTER, FAST, 2, 16>(a, out, c, st);
+    else if (c.th == 8) launch_fused_t<COUNTER, FAST, 2, 8>(a, out, c, st);
+    else launch_fused_t<COUNTER, FAST, 2, 4>(a, out, c, st);
+  } else {
+    if (c.th == 16) launch_fused_t<COUNTER, FAST, 1, 16>(a, out, c, st);
+    else if (c.th == 8) launch_fused_t<COUNTER, FAST, 1, 8>(a, out, c, st);
+    else launch_fused_t<COUNTER, FAST, 1, 4>(a, out, c, st);
+  }
+}
+
+void launch_sweep(sa_ladder *l, const uint64_t *words, uint64_t t) {
   sa::sweep_args a;
   a.s = l->s;
   a.jx = l->jx;
@@ -145,6 +215,16 @@ void launch_sweep(const sa_ladder *l, const uint64_t *words, uint64_t t) {
   a.nsys = l->nsys;
   bool fast = true;
   for (char f : l->fast) fast = fast && f;
+  if (use_fused(l)) {
+    const fused_cfg c = pick_fused(l);
+    if (l->rng_mode == SA_RNG_COUNTER) {
+      if (fast) launch_fused<true, true>(a, l->s_alt, c, l->stream); else launch_fused<true, false>(a, l->s_alt, c, l->stream);
+    } else {
+      if (fast) launch_fused<false, true>(a, l->s_alt, c, l->stream); else launch_fused<false, false>(a, l->s_alt, c, l->stream);
+    }
+    std::swap(l->s, l->s_alt);
+    return;
+  }
   for (int color = 0; color < 2; ++color) {
     if (l->rng_mode == SA_RNG_COUNTER) {
       if (fast) launch_half<true, true>(l, a, color); else launch_half<true, false>(l, a, color);
@@ -386,6 +466,13 @@ int sa_ladder_sweep(sa_ladder *l, int nsweeps) {
   if (!l) return fail(SA_ERR_ARG, "null ladder");
   if (nsweeps < 0) return fail(SA_ERR_ARG, "negative sweep count");
   if (nsweeps == 0) return SA_OK;
+  if (use_fused(l) && !l->s_alt) {
+    const hipError_t e = hipMalloc(&l->s_alt, (size_t)l->nsys * l->N * 8);
+    if (e != hipSuccess) {
+      l->s_alt = nullptr;
+      return fail(SA_ERR_NOMEM, "second spin array: %s", hipGetErrorString(e));
+    }
+  }
   if (l->rng_mode == SA_RNG_COUNTER) {
     for (int k = 0; k < nsweeps; ++k) launch_sweep(l, nullptr, l->t + k);
     SA_HIP_TRY(hipGetLastError());
@@ -412,17 +499,20 @@ int sa_ladder_measure(sa_ladder *l, int32_t *E, int32_t *M) {
   if (!l || !E || !M) return fail(SA_ERR_ARG, "null argument");
   const int n = 128 * l->nsys;
   hipLaunchKernelGGL(sa::k_fill_zero, dim3((n + 255) / 256), dim3(256), 0, l->stream, l->cnt, n);
-  const int64_t blocks = std::min<int64_t>((l->N + 255) / 256, 1024);
-  dim3 grid((unsigned)blocks, l->nsys);
+  // cnt layout [sys][2][64]; one block row per 4 systems, enough blocks for ~2 waves/SIMD
+  const int yb = (l->nsys + 3) / 4;
+  const int64_t want = std::max<int64_t>(1, 2048 / yb);
+  const int64_t blocks = std::min<int64_t>((l->N + 255) / 256, want);
+  dim3 grid((unsigned)blocks, yb);
   if (l->kind == SA_KIND_3D)
     hipLaunchKernelGGL(sa::k_measure<sa::KIND_3D>, grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, l->L,
-                       l->N, l->cnt);
+                       l->N, l->nsys, l->cnt);
   else if (l->kind == SA_KIND_2D)
     hipLaunchKernelGGL(sa::k_measure<sa::KIND_2D>, grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, l->L,
-                       l->N, l->cnt);
+                       l->N, l->nsys, l->cnt);
   else
     hipLaunchKernelGGL(sa::k_measure<sa::KIND_2P>, grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, l->L,
-                       l->N, l->cnt);
+                       l->N, l->nsys, l->cnt);
   SA_HIP_TRY(hipGetLastError());
   SA_HIP_TRY(hipMemcpyAsync(l->cnt_h, l->cnt, sizeof(int) * n, hipMemcpyDeviceToHost, l->stream));
   SA_HIP_TRY(hipStreamSynchronize(l->stream));
@@ -450,19 +540,43 @@ int sa_ladder_swap(sa_ladder *l, int a, int b, uint64_t mask) {
 
 int sa_ladder_set_tuning(sa_ladder *l, int key, int value) {
   if (!l) return fail(SA_ERR_ARG, "null ladder");
-  if (key != SA_TUNE_SYS_PER_THREAD) return fail(SA_ERR_ARG, "unknown tuning key %d", key);
-  if (value != 0 && value != 1 && value != 2 && value != 4)
-    return fail(SA_ERR_ARG, "systems per thread must be 0 (auto), 1, 2 or 4");
-  if (value > 0 && l->nsys % value != 0) return fail(SA_ERR_ARG, "%d systems per thread do not divide nsys=%d", value, l->nsys);
-  l->sys_per_thread = value;
-  return SA_OK;
+  switch (key) {
+    case SA_TUNE_SYS_PER_THREAD:
+      if (value != 0 && value != 1 && value != 2 && value != 4)
+        return fail(SA_ERR_ARG, "systems per thread must be 0 (auto), 1, 2 or 4");
+      if (value > 0 && l->nsys % value != 0)
+        return fail(SA_ERR_ARG, "%d systems per thread do not divide nsys=%d", value, l->nsys);
+      l->sys_per_thread = value;
+      return SA_OK;
+    case SA_TUNE_SWEEP3D_KERNEL:
+      if (value < 0 || value > 2) return fail(SA_ERR_ARG, "3D sweep kernel must be 0 (auto), 1 or 2");
+      l->sweep3d_kernel = value;
+      return SA_OK;
+    case SA_TUNE_ROWS_PER_TILE:
+      if (value != 0 && value != 4 && value != 8 && value != 16)
+        return fail(SA_ERR_ARG, "rows per tile must be 0 (auto), 4, 8 or 16");
+      l->rows_per_tile = value;
+      return SA_OK;
+    case SA_TUNE_XCD_MAP:
+      if (value < -1 || value > 1) return fail(SA_ERR_ARG, "XCD map must be -1 (auto), 0 or 1");
+      l->xcd_map = value;
+      return SA_OK;
+    default:
+      return fail(SA_ERR_ARG, "unknown tuning key %d", key);
+  }
 }
 
 int sa_ladder_get_tuning(const sa_ladder *l, int key, int *value) {
   if (!l || !value) return fail(SA_ERR_ARG, "null argument");
-  if (key != SA_TUNE_SYS_PER_THREAD) return fail(SA_ERR_ARG, "unknown tuning key %d", key);
-  *value = pick_mb(l);
-  return SA_OK;
+  const bool fused = use_fused(l);
+  const fused_cfg c = fused ? pick_fused(l) : fused_cfg{0, 0, 0};
+  switch (key) {
+    case SA_TUNE_SYS_PER_THREAD: *value = fused ? c.mb : pick_mb(l); return SA_OK;
+    case SA_TUNE_SWEEP3D_KERNEL: *value = l->kind != SA_KIND_3D ? 0 : (fused ? 2 : 1); return SA_OK;
+    case SA_TUNE_ROWS_PER_TILE: *value = c.th; return SA_OK;
+    case SA_TUNE_XCD_MAP: *value = c.xcd; return SA_OK;
+    default: return fail(SA_ERR_ARG, "unknown tuning key %d", key);
+  }
 }
 
 int sa_ladder_sync(sa_ladder *l) {

@@ -236,3 +236,57 @@ def test_unsupported_geometry_is_an_error(sa):
         sa.Ladder(2, 7, 1)
     with pytest.raises(sa.SAError):
         sa.Ladder(1, 2, 1)
+
+
+# ---- every 3D kernel variant against the oracle ----------------------------------------
+@pytest.mark.parametrize("L,nsys,kernel,th,mb,xcd", [
+    (8, 1, 2, 8, 1, -1), (8, 2, 2, 4, 2, 0), (16, 4, 2, 16, 2, -1), (16, 2, 2, 8, 1, 1), (32, 4, 2, 8, 2, 1),
+    (32, 2, 1, 0, 2, -1), (64, 2, 2, 16, 1, 0), (16, 4, 2, 4, 1, 1), (4, 2, 2, 4, 1, -1), (32, 6, 2, 4, 2, -1)])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_3d_kernel_variants_match_oracle(sa, orc, L, nsys, kernel, th, mb, xcd, mode):
+    say("3d variant", "L", L, "nsys", nsys, "kernel", kernel, "th", th, "mb", mb, "xcd", xcd, "mode", mode)
+    kind, seed, nsw = 2, 31 + L, 3
+    lad = sa.Ladder(kind, L, nsys)
+    lad.set_rng(mode)
+    lad.tune(sa.TUNE_SWEEP3D_KERNEL, kernel)
+    lad.tune(sa.TUNE_SYS_PER_THREAD, mb)
+    lad.tune(sa.TUNE_ROWS_PER_TILE, th)
+    lad.tune(sa.TUNE_XCD_MAP, xcd)
+    lad.seed_like_main(seed)
+    Ts = [0.4 + 0.5 * j for j in range(nsys)]
+    hs = [0.0 if j % 3 else 0.25 for j in range(nsys)] if mode == 1 else [0.0] * nsys
+    for j in range(nsys):
+        lad.set_temperature(j, Ts[j], hs[j])
+    assert lad.tuning(sa.TUNE_SWEEP3D_KERNEL) == kernel
+    if kernel == 2:
+        assert lad.tuning(sa.TUNE_ROWS_PER_TILE) == th
+    jx, jy, jz, s0, g = orc.init_system(kind, L, seed)
+    ref = [s0.copy() for _ in range(nsys)]
+    for t in range(nsw):
+        w = orc.counter_words(kind, L, seed, t) if mode == 1 else g.fill(L**3)
+        for j in range(nsys):
+            orc.sweep(kind, L, ref[j], jx, jy, jz, orc.boltz(kind, Ts[j], hs[j]), w)
+    lad.sweep(nsw)
+    for j in range(nsys):
+        assert (lad.spins(j) == ref[j]).all(), j
+    E, M = lad.measure()
+    for j in range(nsys):
+        e, m = orc.measure(kind, L, ref[j], jx, jy, jz)
+        assert (E[j] == e).all() and (M[j] == m).all()
+
+
+def test_fused_and_half_sweep_agree_at_L128(sa):
+    say("fused vs half-sweep, 3d L128 x 4 systems")
+    out = []
+    for kernel in (1, 2):
+        lad = sa.Ladder(2, 128, 4)
+        lad.set_rng(sa.RNG_COUNTER)
+        lad.tune(sa.TUNE_SWEEP3D_KERNEL, kernel)
+        lad.seed_like_main(17)
+        for j, t in enumerate([0.5, 0.8, 1.2, 2.0]):
+            lad.set_temperature(j, t)
+        lad.sweep(4)
+        out.append([lad.spins(j) for j in range(4)])
+        lad.close()
+    for a, b in zip(*out):
+        assert (a == b).all()
