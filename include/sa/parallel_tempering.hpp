@@ -69,26 +69,36 @@ class parallel_tempering {
   // src/parallel_tempering.hpp:81-115
   void step(int stride) {
     const bool grouped = stride == 0 && all_on_ladders();
+    int i;
+    double u;
     if (grouped) {
-      sweep_all(steps_);  // whole step at once: one launch sequence per ladder
-      measure_all();
+      // Whole step at once: one launch sequence per ladder.  With stride 0 the step
+      // uses only the coldest system's energies (E_min) and the exchanging pair's;
+      // the pair is drawn first (the draws do not depend on the energies), those
+      // systems are measured now and the others when E is next read -- their spins
+      // do not change before that, so the values are the reference's.
+      sweep_all(steps_);
+      i = uniform_int(gen_, uni_a_, uni_b_);
+      u = canonical32(gen_);
+      measure_some({0, i, i + 1});
       min_E_update();
     } else {
-      for (long i = 0; i < steps_; ++i) {
+      for (long s = 0; s < steps_; ++s) {
         sweep_all(1);
-        if (((stride != 0) && ((i % stride) == 0)) || i == (steps_ - 1)) measure_all();
-        if (stride != 0 && (i % stride) == 0) print(PRINT_E);
+        if (((stride != 0) && ((s % stride) == 0)) || s == (steps_ - 1)) measure_all();
+        if (stride != 0 && (s % stride) == 0) print(PRINT_E);
         else if (stride == 0) min_E_update();
       }
+      i = uniform_int(gen_, uni_a_, uni_b_);
+      u = canonical32(gen_);
     }
-    const int i = uniform_int(gen_, uni_a_, uni_b_);
-    const double u = canonical32(gen_);
     std::unique_ptr<spin_t[]> mask = creat_mask(i, u);
     if (log_) log_->push_back(swap_record{i, u, (uint64_t)mask[0]});
     sys_[i]->swap(sys_[i + 1], std::move(mask));
   }
 
   void print(print_select select) {
+    refresh();
     open_file(select, false);
     if (select == PRINT_E) {
       for (size_t i = 0; i < T_.size(); ++i) {
@@ -117,7 +127,10 @@ class parallel_tempering {
     for (auto &a : num_switch_) a.reset();
   }
   std::vector<float> get_E_min() const { return E_min_; }
-  const std::vector<std::vector<float>> &get_E() const { return E_; }
+  const std::vector<std::vector<float>> &get_E() {
+    refresh();
+    return E_;
+  }
   const std::vector<double> &get_T() const { return T_; }
   std::vector<double> acceptance() const {
     std::vector<double> r;
@@ -130,6 +143,8 @@ class parallel_tempering {
   // Insert a temperature between neighbours whose exchange rate is below 0.3,
   // cloning the hotter neighbour (src/parallel_tempering.hpp:381-413).
   void update_T() {
+    refresh();
+    stale_.clear();
     for (size_t i = 0; i < num_switch_.size(); i++) {
       if (num_switch_[i].get() < 0.3) {
         i++;
@@ -148,6 +163,7 @@ class parallel_tempering {
 
   // field order of parallel_tempering::save (src/parallel_tempering.hpp:280-315)
   std::ostream &save(std::ostream &os) {
+    refresh();
     auto w = [&](const void *p, size_t n) { os.write(reinterpret_cast<const char *>(p), n); };
     w(&steps_, sizeof steps_);
     w(&print_E_, sizeof print_E_);
@@ -271,7 +287,48 @@ class parallel_tempering {
       }
     }
   }
+  // energies of the listed systems now; the rest become stale until refresh()
+  void measure_some(std::vector<int> want) {
+    std::sort(want.begin(), want.end());
+    want.erase(std::unique(want.begin(), want.end()), want.end());
+    stale_.assign(sys_.size(), 1);
+    measure_list(want);
+  }
+  void refresh() {
+    std::vector<int> want;
+    for (size_t j = 0; j < stale_.size() && j < sys_.size(); ++j)
+      if (stale_[j]) want.push_back((int)j);
+    if (!want.empty()) measure_list(want);
+  }
+  void measure_list(const std::vector<int> &want) {
+    std::map<ladder *, std::vector<int>> by;  // ladder -> driver indices
+    for (int j : want) {
+      hip_sys *h = as_hip(sys_[j]);
+      if (!h || h->get_ladder()->nsys() == 1) {
+        E_[j] = sys_[j]->measure();
+        if ((size_t)j < stale_.size()) stale_[j] = 0;
+      } else {
+        by[h->get_ladder().get()].push_back(j);
+      }
+    }
+    for (auto &kv : by) {
+      std::vector<int> slots;
+      for (int j : kv.second) slots.push_back(as_hip(sys_[j])->index());
+      std::vector<int32_t> E(slots.size() * 64), M(slots.size() * 64);
+      const int rc = sa_ladder_measure_systems(kv.first->get(), slots.data(), (int)slots.size(), E.data(), M.data());
+      if (rc != SA_OK) log_error("measure", rc);
+      for (size_t q = 0; q < kv.second.size(); ++q) {
+        const int j = kv.second[q];
+        const double h = as_hip(sys_[j])->get_h();
+        std::vector<float> r(64);
+        for (int k = 0; k < 64; ++k) r[k] = (float)((double)E[q * 64 + k] + h * (double)M[q * 64 + k]);
+        E_[j] = r;
+        if ((size_t)j < stale_.size()) stale_[j] = 0;
+      }
+    }
+  }
   void measure_all() {
+    stale_.assign(sys_.size(), 0);
     std::map<ladder *, std::pair<std::vector<int32_t>, std::vector<int32_t>>> got;
     for (size_t j = 0; j < sys_.size(); ++j) {
       hip_sys *h = as_hip(sys_[j]);
@@ -329,6 +386,7 @@ class parallel_tempering {
   int uni_a_, uni_b_;
   bool print_E_ = false, print_min_ = false;
   std::vector<swap_record> *log_ = nullptr;
+  std::vector<char> stale_;  // E entries not measured at the last step (see step())
   std::vector<std::unique_ptr<gpu_sys>> owned_;
 };
 

@@ -113,6 +113,10 @@ int sa_ladder_sweep(sa_ladder *l, int nsweeps);
  * The reference's measure() returns float(E + h*M) at index k
  * (src/cpu_3d.cpp:310-371; GPU src/gpu_3d.cu:168-203).  Synchronous. */
 int sa_ladder_measure(sa_ladder *l, int32_t *E, int32_t *M);
+/* The same for the n systems listed in sys[] (E[j*64+k] for the j-th listed one):
+ * the replica-exchange step needs only the exchanging pair and the coldest system
+ * (src/parallel_tempering.hpp:91-114). */
+int sa_ladder_measure_systems(sa_ladder *l, const int *sys, int n, int32_t *E, int32_t *M);
 
 /* Exchange the lanes selected by `mask` between systems a and b
  * (swap, src/gpu_3d.cu:310-328, src/cpu_3d.cpp:429-444). */

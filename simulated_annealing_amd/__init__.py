@@ -28,7 +28,7 @@ EXPORTS = [
     "sa_ladder_get_engine", "sa_ladder_set_engine",
     "sa_ladder_init_couplings", "sa_ladder_init_spins", "sa_ladder_set_couplings", "sa_ladder_get_couplings",
     "sa_ladder_set_spins", "sa_ladder_get_spins", "sa_ladder_set_temperature", "sa_ladder_sweep",
-    "sa_ladder_measure", "sa_ladder_swap", "sa_ladder_sync", "sa_ladder_stream",
+    "sa_ladder_measure", "sa_ladder_measure_systems", "sa_ladder_swap", "sa_ladder_sync", "sa_ladder_stream",
     "sa_ladder_set_tuning", "sa_ladder_get_tuning",
     "sa_pt_run", "sa_anneal_run",
     "sa_pt_create", "sa_pt_step", "sa_pt_sync", "sa_pt_steps_done", "sa_pt_result", "sa_pt_ladder",
@@ -79,6 +79,7 @@ def lib():
         L.sa_ladder_set_temperature.argtypes = [_vp, C.c_int, C.c_double, C.c_double, _vp]
         L.sa_ladder_sweep.argtypes = [_vp, C.c_int]
         L.sa_ladder_measure.argtypes = [_vp, _i32p, _i32p]
+        L.sa_ladder_measure_systems.argtypes = [_vp, _i32p, C.c_int, _i32p, _i32p]
         L.sa_ladder_swap.argtypes = [_vp, C.c_int, C.c_int, C.c_uint64]
         L.sa_ladder_sync.argtypes = [_vp]
         L.sa_ladder_stream.argtypes = [_vp]
@@ -217,6 +218,14 @@ class Ladder:
         M = np.zeros(self.nsys * 64, dtype=np.int32)
         check(lib().sa_ladder_measure(self.h, E, M), "measure")
         return E.reshape(self.nsys, 64), M.reshape(self.nsys, 64)
+
+    def measure_systems(self, systems):
+        """(E, M) [len(systems), 64] for the listed systems only."""
+        idx = np.ascontiguousarray(systems, dtype=np.int32)
+        E = np.zeros(idx.size * 64, dtype=np.int32)
+        M = np.zeros(idx.size * 64, dtype=np.int32)
+        check(lib().sa_ladder_measure_systems(self.h, idx, idx.size, E, M), "measure_systems")
+        return E.reshape(-1, 64), M.reshape(-1, 64)
 
     def swap(self, a: int, b: int, mask: int):
         check(lib().sa_ladder_swap(self.h, a, b, C.c_uint64(mask & 0xFFFFFFFFFFFFFFFF)), "swap")

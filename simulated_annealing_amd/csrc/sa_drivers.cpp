@@ -82,7 +82,7 @@ int sa_pt_result(sa_pt *p, int64_t capacity, int32_t *pairs, double *us, uint64_
     if (us) us[k] = p->log[k].u;
     if (masks) masks[k] = p->log[k].mask;
   }
-  const auto &E = p->pt->get_E();
+  const auto &E = p->pt->get_E();  // measures systems not measured at the last step
   if (E_last)
     for (size_t j = 0; j < p->sys.size(); ++j)
       for (int k = 0; k < 64; ++k) E_last[j * 64 + k] = E[j].size() == 64 ? E[j][k] : 0.f;

@@ -298,13 +298,12 @@ __global__ void __launch_bounds__(256) k_sweep3d_half(sweep_args p, int color) {
 // site of pair k is x0 = 2k+P and the colour-1 site x1 = 2k+1-P, so
 //   colour-0 x-neighbours: c1[k], c1[k + (P ? 1 : -1)]   y-neighbours: c1[k] of rows r-1, r+1
 //   colour-1 x-neighbours: c0[k], c0[k + (P ? -1 : 1)]   y-neighbours: c0[k] of rows r-1, r+1
-// Two barriers per plane; the plane buffers alternate so the next plane's writes
-// never meet the current plane's reads.  A thread's row parity is fixed, so the
-// z-walk is instantiated per parity when a row spans whole waves (L/2 a multiple
-// of 64), where the choice is wave-uniform; smaller lattices keep the parity as a
-// run-time value.  z advances two planes per
-// iteration (one Philox block serves a column's pair of planes) with two load
-// buffers used alternately.
+// One barrier per plane: iteration z updates colour 1 of plane z and colour 0 of
+// plane z+1 (which reads only old colour-1 words of plane z+1 and the column's own
+// colour-0 chain) and stores the old words of plane z+2; three LDS plane buffers
+// rotate.  A thread's row parity is fixed, so the z-walk is instantiated per
+// parity when a row spans whole waves (L/2 a multiple of 64), where the choice is
+// wave-uniform; smaller lattices keep the parity as a run-time value.
 template <int MB>
 struct fused_loads {
   uint64_t na[MB], nb[MB];  // own pair at plane z+1 (old)
@@ -330,7 +329,7 @@ struct fused_ctx {
 
   __device__ __forceinline__ uint64_t &c0(int buf, int rr, int m, int kk) const {
     return lds[(((buf * (TH + 2) + rr) * MB + m) * 2 + 0) * PR + kk];
-  }
+  }  // buf = plane % 3
   __device__ __forceinline__ uint64_t &c1(int buf, int rr, int m, int kk) const {
     return lds[(((buf * (TH + 2) + rr) * MB + m) * 2 + 1) * PR + kk];
   }
@@ -359,74 +358,112 @@ struct fused_ctx {
     }
   }
 
-  // one plane: both halves.  pa/pb: the pair's old words at plane z (in),
-  // replaced by the old words at plane z+1 (out).
+  // What the colour-1 update of a plane needs, captured right after the colour-0
+  // update of the same plane.
+  struct bstate {
+    uint64_t jw, je, jn, js, jd, g;
+    uint64_t si[MB], down[MB], n0[MB];
+  };
+
+  // colour-0 update of plane q (all rows, halo included).  q_loads holds the loads
+  // of plane q, pair[] the old pair at plane q.  Writes the new colour-0 words into
+  // LDS buffer q%3 and fills bs for the colour-1 update of plane q.
   template <int PT>
-  __device__ __forceinline__ void plane(int z, const fused_loads<MB> &cur, uint64_t g0, uint64_t g1, uint64_t *pa,
-                                        uint64_t *pb, uint64_t *up0, uint64_t *up1, uint64_t *first0,
-                                        uint64_t *first1, uint64_t &ju0, uint64_t &ju1) const {
+  __device__ __forceinline__ void phase_a(int q, const fused_loads<MB> &ld, const uint64_t *pa, const uint64_t *pb,
+                                          uint64_t g0, uint64_t g1, uint64_t *up0, uint64_t *first0,
+                                          const uint64_t *first1, uint64_t &ju0, bstate &bs) const {
     const int P = PT >= 0 ? PT : par;
-    const int buf = z & 1;
-    // colour-0 word = element P of the pair, colour-1 word = element 1-P
-#pragma unroll
-    for (int m = 0; m < MB; ++m) {
-      c0(buf, r, m, k) = P ? pb[m] : pa[m];
-      c1(buf, r, m, k) = P ? pa[m] : pb[m];
-    }
-    const uint64_t j0w = P ? cur.jxb : cur.jxa, j0e = P ? cur.jxn : cur.jxb;
-    const uint64_t j1w = P ? cur.jxa : cur.jxb, j1e = P ? cur.jxb : cur.jxn;
-    const uint64_t j0n = P ? cur.jyb : cur.jya, j0s = P ? cur.jsb : cur.jsa;
-    const uint64_t j1n = P ? cur.jya : cur.jyb, j1s = P ? cur.jsa : cur.jsb;
-    const uint64_t j0d = P ? cur.jdb : cur.jda, j1d = P ? cur.jda : cur.jdb;
-    const bool last = z == L - 1;
-    __syncthreads();
-    // ---- offset-0 half: colour-0 site of every row, halo rows included
-    uint64_t n0[MB];
+    const int buf = q % 3;
+    const bool last = q == L - 1;
+    const uint64_t j0w = P ? ld.jxb : ld.jxa, j0e = P ? ld.jxn : ld.jxb;
+    const uint64_t j0n = P ? ld.jyb : ld.jya, j0s = P ? ld.jsb : ld.jsa;
+    const uint64_t j0d = P ? ld.jdb : ld.jda;
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
       const uint64_t si = P ? pb[m] : pa[m];
       const uint64_t w = c1(buf, r, m, P ? k : kp);
       const uint64_t e = c1(buf, r, m, P ? kn : k);
-      const uint64_t nn = (r == 0) ? cur.ring2[m] : c1(buf, r - 1, m, k);
-      const uint64_t ss = (r == TH + 1) ? cur.ring2[m] : c1(buf, r + 1, m, k);
-      const uint64_t down = last ? first0[m] : (P ? cur.nb[m] : cur.na[m]);
+      const uint64_t nn = (r == 0) ? ld.ring2[m] : c1(buf, r - 1, m, k);
+      const uint64_t ss = (r == TH + 1) ? ld.ring2[m] : c1(buf, r + 1, m, k);
+      const uint64_t down = last ? first0[m] : (P ? ld.nb[m] : ld.na[m]);
       const count3 c = count6(xor3(j0w, si, w), xor3(j0e, si, e), xor3(j0n, si, nn), xor3(j0s, si, ss),
                               xor3(ju0, si, up0[m]), xor3(j0d, si, down));
       const sys_table &t = p.tab[sys0 + m];
-      n0[m] = FAST ? update_fast3(t, g0, c, si) : update_general(t, 0, g0, c, si, 7);
-      c0(buf, r, m, k) = n0[m];
-      up0[m] = n0[m];
-      if (z == 0) first0[m] = n0[m];
-    }
-    __syncthreads();
-    // ---- offset-1 half: colour-1 site of the band rows
-    if (!halo) {
-#pragma unroll
-      for (int m = 0; m < MB; ++m) {
-        const uint64_t si = P ? pa[m] : pb[m];
-        const uint64_t w = c0(buf, r, m, P ? kp : k);
-        const uint64_t e = c0(buf, r, m, P ? k : kn);
-        const uint64_t nn = c0(buf, r - 1, m, k);
-        const uint64_t ss = c0(buf, r + 1, m, k);
-        const uint64_t down = last ? first1[m] : (P ? cur.na[m] : cur.nb[m]);
-        const count3 c = count6(xor3(j1w, si, w), xor3(j1e, si, e), xor3(j1n, si, nn), xor3(j1s, si, ss),
-                                xor3(ju1, si, up1[m]), xor3(j1d, si, down));
-        const sys_table &t = p.tab[sys0 + m];
-        const uint64_t n1 = FAST ? update_fast3(t, g1, c, si) : update_general(t, 0, g1, c, si, 7);
-        up1[m] = n1;
-        if (z == 0) first1[m] = n1;
-        bst2(sout[m], vpair, (uint32_t)z * plane_bytes, P ? n1 : n0[m], P ? n0[m] : n1);
-      }
+      const uint64_t n0 = FAST ? update_fast3(t, g0, c, si) : update_general(t, 0, g0, c, si, 7);
+      c0(buf, r, m, k) = n0;
+      up0[m] = n0;
+      if (q == 0) first0[m] = n0;
+      bs.si[m] = P ? pa[m] : pb[m];
+      bs.down[m] = last ? first1[m] : (P ? ld.na[m] : ld.nb[m]);
+      bs.n0[m] = n0;
     }
     ju0 = j0d;
-    ju1 = j1d;
+    bs.jw = P ? ld.jxa : ld.jxb;
+    bs.je = P ? ld.jxb : ld.jxn;
+    bs.jn = P ? ld.jya : ld.jyb;
+    bs.js = P ? ld.jsa : ld.jsb;
+    bs.jd = P ? ld.jda : ld.jdb;
+    bs.g = g1;
+  }
+
+  // colour-1 update of plane q (band rows): reads the new colour-0 words of plane q
+  // from LDS buffer q%3 and stores the pair's new words.
+  template <int PT>
+  __device__ __forceinline__ void phase_b(int q, const bstate &bs, uint64_t *up1, uint64_t *first1,
+                                          uint64_t &ju1) const {
+    const int P = PT >= 0 ? PT : par;
+    const int buf = q % 3;
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
-      pa[m] = cur.na[m];
-      pb[m] = cur.nb[m];
+      const uint64_t si = bs.si[m];
+      const uint64_t w = c0(buf, r, m, P ? kp : k);
+      const uint64_t e = c0(buf, r, m, P ? k : kn);
+      const uint64_t nn = c0(buf, r - 1, m, k);
+      const uint64_t ss = c0(buf, r + 1, m, k);
+      const count3 c = count6(xor3(bs.jw, si, w), xor3(bs.je, si, e), xor3(bs.jn, si, nn), xor3(bs.js, si, ss),
+                              xor3(ju1, si, up1[m]), xor3(bs.jd, si, bs.down[m]));
+      const sys_table &t = p.tab[sys0 + m];
+      const uint64_t n1 = FAST ? update_fast3(t, bs.g, c, si) : update_general(t, 0, bs.g, c, si, 7);
+      up1[m] = n1;
+      if (q == 0) first1[m] = n1;
+      bst2(sout[m], vpair, (uint32_t)q * plane_bytes, P ? n1 : bs.n0[m], P ? bs.n0[m] : n1);
+    }
+    ju1 = bs.jd;
+  }
+
+  // old pair of plane q into LDS buffer q%3 (colour-0 and colour-1 slots)
+  template <int PT>
+  __device__ __forceinline__ void put_old(int q, const uint64_t *pa, const uint64_t *pb) const {
+    const int P = PT >= 0 ? PT : par;
+    const int buf = q % 3;
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      c0(buf, r, m, k) = P ? pb[m] : pa[m];
+      c1(buf, r, m, k) = P ? pa[m] : pb[m];
     }
   }
 
+  // random words of the colour-0 / colour-1 site at plane q
+  __device__ __forceinline__ void words(int q, const fused_loads<MB> &ld, int x0, int x1, u32x4 &blk0, u32x4 &blk1,
+                                        uint64_t &g0, uint64_t &g1) const {
+    if (COUNTER) {
+      if ((q & 1) == 0) {
+        blk0 = counter_block((uint32_t)(x0 + row + q * L2), p.seed, p.sweep);
+        if (!halo) blk1 = counter_block((uint32_t)(x1 + row + q * L2), p.seed, p.sweep);
+      }
+      g0 = counter_half(blk0, q & 1);
+      g1 = counter_half(blk1, q & 1);
+    } else {
+      g0 = ld.g0;
+      g1 = ld.g1;
+    }
+  }
+
+  // One pass over z.  Iteration z: colour-1 update of plane z, colour-0 update of
+  // plane z+1, old words of plane z+2 into LDS, one barrier.  Phase A(z+1) may run
+  // before phase B(z) finishes elsewhere: it reads only old colour-1 words of plane
+  // z+1 and the column's own colour-0 chain.  Three LDS plane buffers: z (read by
+  // phase B), z+1 (phase A), z+2 (being filled).
   template <int PT>
   __device__ __forceinline__ void walk() const {
     const int P = PT >= 0 ? PT : par;
@@ -445,27 +482,46 @@ struct fused_ctx {
     uint64_t ja, jb;
     bld2(bjz, vpair, 0u, ja, jb);  // -z bonds at plane 0
     uint64_t ju0 = P ? jb : ja, ju1 = P ? ja : jb;
-    fused_loads<MB> A, B;
+    u32x4 blk0{0, 0, 0, 0}, blk1{0, 0, 0, 0};
+    fused_loads<MB> A, B;  // loads of planes z+1 / z+2, used alternately
+    bstate bs;
+    // prologue: old planes 0 and 1 into LDS, colour-0 update of plane 0
     load(0, A);
-    for (int z = 0; z < L; z += 2) {  // L is even
-      load(z + 1, B);
-      uint64_t g0a, g1a, g0b, g1b;
-      if (COUNTER) {
-        const u32x4 blk0 = counter_block((uint32_t)(x0 + row + z * L2), p.seed, p.sweep);
-        const u32x4 blk1 = halo ? blk0 : counter_block((uint32_t)(x1 + row + z * L2), p.seed, p.sweep);
-        g0a = counter_half(blk0, 0);
-        g0b = counter_half(blk0, 1);
-        g1a = counter_half(blk1, 0);
-        g1b = counter_half(blk1, 1);
-      } else {
-        g0a = A.g0;
-        g1a = A.g1;
-        g0b = B.g0;
-        g1b = B.g1;
+    load(1, B);
+    put_old<PT>(0, pa, pb);
+    put_old<PT>(1, A.na, A.nb);
+    __syncthreads();
+    {
+      uint64_t g0, g1;
+      words(0, A, x0, x1, blk0, blk1, g0, g1);
+      phase_a<PT>(0, A, pa, pb, g0, g1, up0, first0, first1, ju0, bs);
+    }
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      pa[m] = A.na[m];  // old pair at plane 1
+      pb[m] = A.nb[m];
+    }
+    __syncthreads();
+    // iteration z: near = loads of plane z+1, far = loads of plane z+2 (prefetched)
+    auto iteration = [&](int z, fused_loads<MB> &near, fused_loads<MB> &far) {
+      if (z + 2 < L) load(z + 2, far);
+      if (!halo) phase_b<PT>(z, bs, up1, first1, ju1);
+      if (z + 1 < L) {
+        uint64_t g0, g1;
+        words(z + 1, near, x0, x1, blk0, blk1, g0, g1);
+        phase_a<PT>(z + 1, near, pa, pb, g0, g1, up0, first0, first1, ju0, bs);
+        if (z + 2 < L) put_old<PT>(z + 2, near.na, near.nb);
+#pragma unroll
+        for (int m = 0; m < MB; ++m) {
+          pa[m] = near.na[m];  // old pair at plane z+2
+          pb[m] = near.nb[m];
+        }
       }
-      plane<PT>(z, A, g0a, g1a, pa, pb, up0, up1, first0, first1, ju0, ju1);
-      if (z + 2 < L) load(z + 2, A);
-      plane<PT>(z + 1, B, g0b, g1b, pa, pb, up0, up1, first0, first1, ju0, ju1);
+      __syncthreads();
+    };
+    for (int z = 0; z < L; z += 2) {  // L is even
+      iteration(z, B, A);
+      iteration(z + 1, A, B);
     }
   }
 };
@@ -608,7 +664,8 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t x, int lane) {
 template <int KIND>
 __global__ void __launch_bounds__(256) k_measure(const uint64_t *__restrict__ s_all, const uint64_t *__restrict__ jx,
                                                  const uint64_t *__restrict__ jy, const uint64_t *__restrict__ jz,
-                                                 int L, int64_t N, int nsys, int *cnt) {
+                                                 int L, int64_t N, int nsys, const int *__restrict__ sys_list,
+                                                 int *cnt) {
   constexpr int FLUSH = 16;  // groups between flushes: E <= 3*16 < 64, M <= 16 < 32
   constexpr int SB = 4;      // systems per pass over the sites (couplings loaded once per pass)
   __shared__ int red[SB][2][64];
@@ -677,7 +734,8 @@ __global__ void __launch_bounds__(256) k_measure(const uint64_t *__restrict__ s_
 #pragma unroll
     for (int m = 0; m < SB; ++m) {
       if (m >= ns) break;
-      const uint64_t *s = s_all + (int64_t)(sys0 + m) * N;
+      const int sys = sys_list ? sys_list[sys0 + m] : sys0 + m;
+      const uint64_t *s = s_all + (int64_t)sys * N;
       uint64_t w = 0, bw = 0, bn = 0, bu = 0;
       if (in) {
         w = s[i];

@@ -54,6 +54,7 @@ struct sa_ladder {
   uint64_t *words_d = nullptr, *words_h = nullptr;
   int64_t words_cap = 0;  // sweeps that fit in the staging buffers
   int *cnt_h = nullptr;
+  int *sys_list = nullptr;  // device copy of a measured subset
   int sys_per_thread = 0;  // SA_TUNE_SYS_PER_THREAD, 0 = automatic
   int sweep3d_kernel = 0;  // SA_TUNE_SWEEP3D_KERNEL, 0 = automatic
   int rows_per_tile = 0;   // SA_TUNE_ROWS_PER_TILE, 0 = automatic
@@ -75,6 +76,7 @@ void free_all(sa_ladder *l) {
   if (l->words_d) (void)hipFree(l->words_d);
   if (l->words_h) (void)hipHostFree(l->words_h);
   if (l->cnt_h) (void)hipHostFree(l->cnt_h);
+  if (l->sys_list) (void)hipFree(l->sys_list);
   if (l->stream) (void)hipStreamDestroy(l->stream);
 }
 
@@ -140,7 +142,7 @@ struct fused_cfg {
   int th, mb, xcd;
 };
 
-size_t fused_lds_bytes(int L, int th, int mb) { return (size_t)2 * (th + 2) * mb * 2 * (L / 2) * 8; }
+size_t fused_lds_bytes(int L, int th, int mb) { return (size_t)3 * (th + 2) * mb * 2 * (L / 2) * 8; }  // 3 plane buffers
 
 bool fused_ok(const sa_ladder *l, int th, int mb) {
   if (l->kind != SA_KIND_3D || l->L % th != 0 || th > l->L) return false;
@@ -293,6 +295,7 @@ int sa_ladder_create(int kind, int L, int nsys, sa_ladder **out) {
       bad(hipMalloc(&l->tab, sizeof(sa::sys_table) * nsys), "tables") ||
       bad(hipMalloc(&l->cnt, sizeof(int) * 128 * nsys), "counters") ||
       bad(hipHostMalloc(&l->cnt_h, sizeof(int) * 128 * nsys, hipHostMallocDefault), "pinned counters") ||
+      bad(hipMalloc(&l->sys_list, sizeof(int) * nsys), "system list") ||
       bad(hipMemsetAsync(l->s, 0, (size_t)nsys * N * 8, l->stream), "memset") ||
       bad(hipMemsetAsync(l->jx, 0, N * 8, l->stream), "memset") ||
       bad(hipMemsetAsync(l->jy, 0, N * 8, l->stream), "memset") ||
@@ -495,33 +498,64 @@ int sa_ladder_sweep(sa_ladder *l, int nsweeps) {
   return SA_OK;
 }
 
-int sa_ladder_measure(sa_ladder *l, int32_t *E, int32_t *M) {
-  if (!l || !E || !M) return fail(SA_ERR_ARG, "null argument");
-  const int n = 128 * l->nsys;
-  hipLaunchKernelGGL(sa::k_fill_zero, dim3((n + 255) / 256), dim3(256), 0, l->stream, l->cnt, n);
-  // cnt layout [sys][2][64]; one block row per 4 systems, enough blocks for ~2 waves/SIMD
-  const int yb = (l->nsys + 3) / 4;
+namespace {
+
+// energies/magnetisations of `n` systems (list = nullptr: systems 0..n-1); results
+// land in cnt_h[(2*j)*64 ...] for the j-th listed system
+int measure_list(sa_ladder *l, const int *list, int n) {
+  const int cnt_n = 128 * n;
+  hipLaunchKernelGGL(sa::k_fill_zero, dim3((cnt_n + 255) / 256), dim3(256), 0, l->stream, l->cnt, cnt_n);
+  const int *dlist = nullptr;
+  if (list) {
+    SA_HIP_TRY(hipMemcpyAsync(l->sys_list, list, sizeof(int) * n, hipMemcpyHostToDevice, l->stream));
+    dlist = l->sys_list;
+  }
+  // cnt layout [listed system][2][64]; one block row per 4 systems, ~2 waves/SIMD
+  const int yb = (n + 3) / 4;
   const int64_t want = std::max<int64_t>(1, 2048 / yb);
   const int64_t blocks = std::min<int64_t>((l->N + 255) / 256, want);
   dim3 grid((unsigned)blocks, yb);
   if (l->kind == SA_KIND_3D)
     hipLaunchKernelGGL(sa::k_measure<sa::KIND_3D>, grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, l->L,
-                       l->N, l->nsys, l->cnt);
+                       l->N, n, dlist, l->cnt);
   else if (l->kind == SA_KIND_2D)
     hipLaunchKernelGGL(sa::k_measure<sa::KIND_2D>, grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, l->L,
-                       l->N, l->nsys, l->cnt);
+                       l->N, n, dlist, l->cnt);
   else
     hipLaunchKernelGGL(sa::k_measure<sa::KIND_2P>, grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, l->L,
-                       l->N, l->nsys, l->cnt);
+                       l->N, n, dlist, l->cnt);
   SA_HIP_TRY(hipGetLastError());
-  SA_HIP_TRY(hipMemcpyAsync(l->cnt_h, l->cnt, sizeof(int) * n, hipMemcpyDeviceToHost, l->stream));
+  SA_HIP_TRY(hipMemcpyAsync(l->cnt_h, l->cnt, sizeof(int) * cnt_n, hipMemcpyDeviceToHost, l->stream));
   SA_HIP_TRY(hipStreamSynchronize(l->stream));
+  return SA_OK;
+}
+
+void counts_to_EM(const sa_ladder *l, int j, int32_t *E, int32_t *M) {
   const int64_t nb = bonds_per_site(l->kind) * l->N - (l->kind == SA_KIND_2P ? l->L : 0);
-  for (int i = 0; i < l->nsys; ++i)
-    for (int k = 0; k < 64; ++k) {
-      E[i * 64 + k] = (int32_t)(2 * (int64_t)l->cnt_h[(2 * i) * 64 + k] - nb);
-      M[i * 64 + k] = (int32_t)(2 * (int64_t)l->cnt_h[(2 * i + 1) * 64 + k] - l->N);
-    }
+  for (int k = 0; k < 64; ++k) {
+    E[k] = (int32_t)(2 * (int64_t)l->cnt_h[(2 * j) * 64 + k] - nb);
+    M[k] = (int32_t)(2 * (int64_t)l->cnt_h[(2 * j + 1) * 64 + k] - l->N);
+  }
+}
+
+}  // namespace
+
+int sa_ladder_measure(sa_ladder *l, int32_t *E, int32_t *M) {
+  if (!l || !E || !M) return fail(SA_ERR_ARG, "null argument");
+  const int rc = measure_list(l, nullptr, l->nsys);
+  if (rc) return rc;
+  for (int i = 0; i < l->nsys; ++i) counts_to_EM(l, i, E + i * 64, M + i * 64);
+  return SA_OK;
+}
+
+int sa_ladder_measure_systems(sa_ladder *l, const int *sys, int n, int32_t *E, int32_t *M) {
+  if (!l || !sys || !E || !M) return fail(SA_ERR_ARG, "null argument");
+  if (n < 1 || n > l->nsys) return fail(SA_ERR_ARG, "measure of %d systems out of %d", n, l->nsys);
+  for (int j = 0; j < n; ++j)
+    if (sys[j] < 0 || sys[j] >= l->nsys) return fail(SA_ERR_ARG, "system index %d out of range", sys[j]);
+  const int rc = measure_list(l, sys, n);
+  if (rc) return rc;
+  for (int j = 0; j < n; ++j) counts_to_EM(l, j, E + j * 64, M + j * 64);
   return SA_OK;
 }
 

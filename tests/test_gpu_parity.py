@@ -290,3 +290,20 @@ def test_fused_and_half_sweep_agree_at_L128(sa):
         lad.close()
     for a, b in zip(*out):
         assert (a == b).all()
+
+
+def test_measure_systems_subset_matches_full_measure(sa):
+    say("measure subset, 3d L32 x 6 systems")
+    lad = sa.Ladder(2, 32, 6)
+    lad.set_rng(sa.RNG_COUNTER)
+    lad.seed_like_main(8)
+    for j in range(6):
+        lad.set_temperature(j, 0.5 + 0.3 * j)
+    lad.sweep(3)
+    E, M = lad.measure()
+    for subset in ([0], [5, 1], [2, 3, 4], [0, 1, 2, 3, 4, 5], [4, 4]):
+        Es, Ms = lad.measure_systems(subset)
+        for q, j in enumerate(subset):
+            assert (Es[q] == E[j]).all() and (Ms[q] == M[j]).all()
+    with pytest.raises(sa.SAError):
+        lad.measure_systems([6])
