@@ -108,6 +108,11 @@ int sa_ladder_set_temperature(sa_ladder *l, int sys, double T, double h, double 
  * consumed. */
 int sa_ladder_sweep(sa_ladder *l, int nsweeps);
 
+/* An anneal on the ladder: before sweep t, system j gets temperature T[t*nsys+j]
+ * and field h, then one sweep (the loop of annealing::run with set_T per sweep,
+ * src/annealing.hpp:67-100, without its printing).  Asynchronous in counter mode. */
+int sa_ladder_anneal(sa_ladder *l, const double *T, int nsweeps, double h);
+
 /* Per system and lane k (bit k): E[sys*64+k] = sum over the -x,-y[,-z] bond of
  * every site of +1 (satisfied) / -1, M[sys*64+k] = sum of +1 (set) / -1 (clear).
  * The reference's measure() returns float(E + h*M) at index k
@@ -176,6 +181,41 @@ int sa_pt_result(sa_pt *pt, int64_t capacity, int32_t *pairs, double *us, uint64
                  uint64_t *spins);
 sa_ladder *sa_pt_ladder(sa_pt *pt);
 int sa_pt_destroy(sa_pt *pt);
+
+/* ---- batched instances (no counterpart in the reference) --------------------
+ * Many independent instances of uneven size and kind, each a ladder seeded like
+ * main_gpu (set_seed, init_J, set_seed, init_rand; src/main_gpu.cu:262-276) with
+ * its own temperatures, swept together: one launch per half-sweep covers every
+ * unfinished instance, chunks of sweeps are replayed as captured hipGraphs, and
+ * block size / chunk length are tuned online.  Counter RNG only.  An instance's
+ * spins after k sweeps are bit for bit those of its ladder swept k times alone
+ * (sa_ladder_set_temperature + sa_ladder_sweep per sweep). */
+typedef struct sa_batch sa_batch;
+int sa_batch_create(sa_batch **out);
+int sa_batch_destroy(sa_batch *b);
+/* Add an instance of nsys systems.  per_sweep = 0: T[nsys] fixed temperatures;
+ * per_sweep = 1: T[nsweeps*nsys], row t = the temperatures of sweep t (an anneal
+ * schedule, annealing::set_T before each sweep, src/annealing.hpp:67-100).
+ * *id receives the instance id. */
+int sa_batch_add(sa_batch *b, int kind, int L, int nsys, uint64_t seed, int nsweeps, const double *T, int per_sweep,
+                 double h, int *id);
+/* Run up to `sweeps` sweeps of every unfinished instance (sweeps <= 0: until all
+ * are done).  Returns the number of unfinished instances (>= 0) or an sa_status. */
+int sa_batch_run(sa_batch *b, int sweeps);
+int sa_batch_sweeps_done(sa_batch *b, int id, int *t);
+int sa_batch_get_spins(sa_batch *b, int id, int sys, uint64_t *s);
+int sa_batch_measure(sa_batch *b, int id, int32_t *E, int32_t *M);
+/* the instance's ladder (owned by the batch; valid until sa_batch_remove) */
+sa_ladder *sa_batch_ladder(sa_batch *b, int id);
+/* free the instance's device memory; its id is not reused */
+int sa_batch_remove(sa_batch *b, int id);
+/* the stream the batch launches on (hipStream_t) */
+void *sa_batch_stream(sa_batch *b);
+/* tuning (no effect on results): threads per block 64/128/256 and sweeps per
+ * captured graph; 0 = picked online by timing the first chunks */
+enum sa_batch_tune_key { SA_TUNE_BATCH_BLOCK = 101, SA_TUNE_BATCH_CHUNK = 102 };
+int sa_batch_set_tuning(sa_batch *b, int key, int value);
+int sa_batch_get_tuning(sa_batch *b, int key, int *value);
 
 #ifdef __cplusplus
 }

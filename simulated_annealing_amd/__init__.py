@@ -19,6 +19,7 @@ KIND_2P, KIND_2D, KIND_3D = 0, 1, 2
 RNG_REPLAY, RNG_COUNTER = 0, 1
 SA_OK = 0
 TUNE_SYS_PER_THREAD, TUNE_SWEEP3D_KERNEL, TUNE_ROWS_PER_TILE, TUNE_XCD_MAP = 1, 2, 3, 4
+TUNE_BATCH_BLOCK, TUNE_BATCH_CHUNK = 101, 102
 
 # every symbol include/sa_hip.h declares (checked by tests/test_capi_symbols.py)
 EXPORTS = [
@@ -28,11 +29,15 @@ EXPORTS = [
     "sa_ladder_get_engine", "sa_ladder_set_engine",
     "sa_ladder_init_couplings", "sa_ladder_init_spins", "sa_ladder_set_couplings", "sa_ladder_get_couplings",
     "sa_ladder_set_spins", "sa_ladder_get_spins", "sa_ladder_set_temperature", "sa_ladder_sweep",
+    "sa_ladder_anneal",
     "sa_ladder_measure", "sa_ladder_measure_systems", "sa_ladder_swap", "sa_ladder_sync", "sa_ladder_stream",
     "sa_ladder_set_tuning", "sa_ladder_get_tuning",
     "sa_pt_run", "sa_anneal_run",
     "sa_pt_create", "sa_pt_step", "sa_pt_sync", "sa_pt_steps_done", "sa_pt_result", "sa_pt_ladder",
     "sa_pt_destroy",
+    "sa_batch_create", "sa_batch_destroy", "sa_batch_add", "sa_batch_run", "sa_batch_sweeps_done",
+    "sa_batch_get_spins", "sa_batch_measure", "sa_batch_ladder", "sa_batch_remove", "sa_batch_stream",
+    "sa_batch_set_tuning", "sa_batch_get_tuning",
 ]
 
 
@@ -78,6 +83,7 @@ def lib():
         L.sa_ladder_get_spins.argtypes = [_vp, C.c_int, _u64p]
         L.sa_ladder_set_temperature.argtypes = [_vp, C.c_int, C.c_double, C.c_double, _vp]
         L.sa_ladder_sweep.argtypes = [_vp, C.c_int]
+        L.sa_ladder_anneal.argtypes = [_vp, _f64p, C.c_int, C.c_double]
         L.sa_ladder_measure.argtypes = [_vp, _i32p, _i32p]
         L.sa_ladder_measure_systems.argtypes = [_vp, _i32p, C.c_int, _i32p, _i32p]
         L.sa_ladder_swap.argtypes = [_vp, C.c_int, C.c_int, C.c_uint64]
@@ -100,6 +106,21 @@ def lib():
         L.sa_pt_ladder.argtypes = [_vp]
         L.sa_pt_ladder.restype = _vp
         L.sa_pt_destroy.argtypes = [_vp]
+        L.sa_batch_create.argtypes = [C.POINTER(_vp)]
+        L.sa_batch_destroy.argtypes = [_vp]
+        L.sa_batch_add.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, _f64p, C.c_int, C.c_double,
+                                   C.POINTER(C.c_int)]
+        L.sa_batch_run.argtypes = [_vp, C.c_int]
+        L.sa_batch_sweeps_done.argtypes = [_vp, C.c_int, C.POINTER(C.c_int)]
+        L.sa_batch_get_spins.argtypes = [_vp, C.c_int, C.c_int, _u64p]
+        L.sa_batch_measure.argtypes = [_vp, C.c_int, _i32p, _i32p]
+        L.sa_batch_ladder.argtypes = [_vp, C.c_int]
+        L.sa_batch_ladder.restype = _vp
+        L.sa_batch_remove.argtypes = [_vp, C.c_int]
+        L.sa_batch_stream.argtypes = [_vp]
+        L.sa_batch_stream.restype = _vp
+        L.sa_batch_set_tuning.argtypes = [_vp, C.c_int, C.c_int]
+        L.sa_batch_get_tuning.argtypes = [_vp, C.c_int, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -212,6 +233,14 @@ class Ladder:
     def sweep(self, n: int = 1):
         check(lib().sa_ladder_sweep(self.h, n), "sweep")
 
+    def anneal(self, schedule, h: float = 0.0):
+        """One sweep per row of `schedule` ([nsweeps] or [nsweeps, nsys]) at those temperatures."""
+        T = np.asarray(schedule, dtype=np.float64)
+        if T.ndim == 1:
+            T = np.repeat(T[:, None], self.nsys, axis=1)
+        T = np.ascontiguousarray(T.reshape(-1, self.nsys))
+        check(lib().sa_ladder_anneal(self.h, T.reshape(-1) if T.size else np.zeros(1), T.shape[0], h), "anneal")
+
     def measure(self):
         """(E, M) int32 arrays [nsys, 64] in lane order."""
         E = np.zeros(self.nsys * 64, dtype=np.int32)
@@ -286,6 +315,90 @@ class PT:
         if getattr(self, "p", None):
             lib().sa_pt_destroy(self.p)
             self.p = None
+
+    def __del__(self):
+        self.close()
+
+
+def linear_schedule(T0: float, T1: float, nsweeps: int) -> np.ndarray:
+    """Temperature of sweep t of a linear anneal from T0 to T1 (t = 0 .. nsweeps-1)."""
+    if nsweeps <= 1:
+        return np.full(max(nsweeps, 0), T0, dtype=np.float64)
+    t = np.arange(nsweeps, dtype=np.float64)
+    return T0 + (T1 - T0) * t / (nsweeps - 1)
+
+
+class Batch:
+    """Independent instances of uneven size swept together (sa_batch_* of include/sa_hip.h)."""
+
+    def __init__(self):
+        b = _vp()
+        check(lib().sa_batch_create(C.byref(b)), "sa_batch_create")
+        self.b = b
+        self.info = {}
+
+    def add(self, kind, L, seed, nsweeps, T=None, schedule=None, nsys=1, h=0.0) -> int:
+        """Add an instance of `nsys` systems seeded like main (set_seed, init_J,
+        set_seed, init_rand).  Give either T: [nsys] fixed temperatures, or
+        schedule: [nsweeps] or [nsweeps, nsys], the temperatures of every sweep."""
+        if (T is None) == (schedule is None):
+            raise ValueError("give exactly one of T and schedule")
+        per_sweep = schedule is not None
+        arr = np.asarray(schedule if per_sweep else T, dtype=np.float64)
+        if per_sweep and arr.ndim == 1:
+            arr = np.repeat(arr[:, None], nsys, axis=1)
+        need = nsweeps * nsys if per_sweep else nsys
+        Tc = np.ascontiguousarray(arr.reshape(-1))
+        if Tc.size != need:
+            raise ValueError(f"expected {need} temperatures, got {Tc.size}")
+        if Tc.size == 0:
+            Tc = np.zeros(1)
+        i = C.c_int()
+        check(lib().sa_batch_add(self.b, kind, L, nsys, seed, nsweeps, Tc, int(per_sweep), h, C.byref(i)),
+              "sa_batch_add")
+        self.info[i.value] = (kind, L, nsys)
+        return i.value
+
+    def run(self, sweeps: int = 0) -> int:
+        return check(lib().sa_batch_run(self.b, sweeps), "sa_batch_run")
+
+    def sweeps_done(self, i: int) -> int:
+        t = C.c_int()
+        check(lib().sa_batch_sweeps_done(self.b, i, C.byref(t)), "sa_batch_sweeps_done")
+        return t.value
+
+    def spins(self, i: int, sys: int = 0) -> np.ndarray:
+        kind, L, _ = self.info[i]
+        s = np.zeros(nsites(kind, L), dtype=np.uint64)
+        check(lib().sa_batch_get_spins(self.b, i, sys, s), "sa_batch_get_spins")
+        return s
+
+    def measure(self, i: int):
+        nsys = self.info[i][2]
+        E = np.zeros(nsys * 64, dtype=np.int32)
+        M = np.zeros(nsys * 64, dtype=np.int32)
+        check(lib().sa_batch_measure(self.b, i, E, M), "sa_batch_measure")
+        return E.reshape(nsys, 64), M.reshape(nsys, 64)
+
+    def remove(self, i: int):
+        check(lib().sa_batch_remove(self.b, i), "sa_batch_remove")
+        self.info.pop(i, None)
+
+    def stream(self):
+        return lib().sa_batch_stream(self.b)
+
+    def tune(self, key: int, value: int):
+        check(lib().sa_batch_set_tuning(self.b, key, value), "sa_batch_set_tuning")
+
+    def tuning(self, key: int) -> int:
+        v = C.c_int()
+        check(lib().sa_batch_get_tuning(self.b, key, C.byref(v)), "sa_batch_get_tuning")
+        return v.value
+
+    def close(self):
+        if getattr(self, "b", None):
+            lib().sa_batch_destroy(self.b)
+            self.b = None
 
     def __del__(self):
         self.close()

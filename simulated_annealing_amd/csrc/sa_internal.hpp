@@ -1,0 +1,51 @@
+// sa_internal.hpp -- definitions shared by the translation units of libsa_hip.so
+// (not part of the public interface).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "sa/rng.hpp"
+#include "sa_hip.h"
+#include "sa_kernels.hpp"
+
+// error reporting: records the message for sa_last_error() and returns `status`
+int sa_fail(int status, const char *fmt, ...);
+
+#define SA_HIP_TRY(expr)                                                                     \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return sa_fail(SA_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+struct sa_ladder {
+  int kind = 0, L = 0, nsys = 0, dim = 0;
+  int64_t N = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t *s = nullptr, *jx = nullptr, *jy = nullptr, *jz = nullptr;
+  uint64_t *s_alt = nullptr;  // second spin array of the fused 3D sweep (ping-pong)
+  sa::sys_table *tab = nullptr;
+  int *cnt = nullptr;
+  std::vector<sa::sys_table> htab;
+  std::vector<char> fast;  // per system: symmetric tables, low levels always accept
+  int rng_mode = SA_RNG_REPLAY;
+  uint64_t seed = 0, t = 0;
+  sa::mt19937_64 eng{};
+  uint64_t *words_d = nullptr, *words_h = nullptr;
+  int64_t words_cap = 0;  // sweeps that fit in the staging buffers
+  int *cnt_h = nullptr;
+  int *sys_list = nullptr;  // device copy of a measured subset
+  int sys_per_thread = 0;  // SA_TUNE_SYS_PER_THREAD, 0 = automatic
+  int sweep3d_kernel = 0;  // SA_TUNE_SWEEP3D_KERNEL, 0 = automatic
+  int rows_per_tile = 0;   // SA_TUNE_ROWS_PER_TILE, 0 = automatic
+  int xcd_map = -1;        // SA_TUNE_XCD_MAP, -1 = automatic
+};
+
+namespace sa_detail {
+// acceptance thresholds of one temperature/field for a ladder of `kind`; fast =
+// the table qualifies for the symmetric low-levels-always kernels
+void make_table(int kind, double T, double h, sa::sys_table &tab, bool &fast, double *boltz_out);
+}  // namespace sa_detail

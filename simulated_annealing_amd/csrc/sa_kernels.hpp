@@ -197,13 +197,11 @@ struct plane_loads {
   uint64_t w[MB], e[MB], n[MB], s[MB], d[MB];
 };
 
+// column k (0 <= k < L*L/2) of colour `color`, systems sys0 .. sys0+MB-1
 template <bool COUNTER, bool FAST, int MB>
-__global__ void __launch_bounds__(256) k_sweep3d_half(sweep_args p, int color) {
+__device__ __forceinline__ void half3d_column(const sweep_args &p, int color, int k, int sys0) {
   const int L = p.L;
   const int half = L >> 1;
-  const int cols = L * half;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= cols) return;
   const int y = k / half;
   const int x = 2 * (k - y * half) + ((y + color) & 1);
   const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
@@ -211,7 +209,6 @@ __global__ void __launch_bounds__(256) k_sweep3d_half(sweep_args p, int color) {
   const int64_t L2 = (int64_t)L * L;
   const int c0 = x + L * y;               // in-plane offsets
   const int cw = xm + L * y, ce = xp + L * y, cn = x + L * ym, cs = x + L * yp;
-  const int sys0 = blockIdx.y * MB;
   const int64_t N = p.N;
   const int64_t rank0 = (int64_t)color * (N >> 1) + k;  // visit rank of (x,y,0)
   const int64_t rstep = L2 >> 1;
@@ -274,6 +271,13 @@ __global__ void __launch_bounds__(256) k_sweep3d_half(sweep_args p, int color) {
     jz_up = cur.jd;
     cur = nxt;
   }
+}
+
+template <bool COUNTER, bool FAST, int MB>
+__global__ void __launch_bounds__(256) k_sweep3d_half(sweep_args p, int color) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= p.L * (p.L >> 1)) return;
+  half3d_column<COUNTER, FAST, MB>(p, color, k, blockIdx.y * MB);
 }
 
 // ---------------------------------------------------------------- 3D fused sweep
@@ -591,19 +595,17 @@ __global__ void __launch_bounds__(640) k_sweep3d_fused(sweep_args p, uint64_t *_
 }
 
 // ---------------------------------------------------------------- 2D half-sweep
+// site k (0 <= k < L*L/2) of colour `color`, systems sys0 .. sys0+MB-1
 template <bool COUNTER, bool FAST, bool OPEN, int MB>
-__global__ void __launch_bounds__(256) k_sweep2d_half(sweep_args p, int color) {
+__device__ __forceinline__ void half2d_site(const sweep_args &p, int color, int k, int sys0) {
   const int L = p.L;
   const int half = L >> 1;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= L * half) return;
   const int y = k / half;
   const int x = 2 * (k - y * half) + ((y + color) & 1);
   const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
   const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
   const int c0 = x + L * y;
   const int cw = xm + L * y, ce = xp + L * y, cn = x + L * ym, cs = x + L * yp;
-  const int sys0 = blockIdx.y * MB;
   const uint64_t jw = p.jx[c0], je = p.jx[ce], jn = p.jy[c0], js = p.jy[cs];
   uint64_t g;
   if (COUNTER) {
@@ -629,6 +631,13 @@ __global__ void __launch_bounds__(256) k_sweep2d_half(sweep_args p, int color) {
     const sys_table &t = p.tab[sys0 + m];
     s[c0] = (FAST && !OPEN) ? update_fast2(t, g, c, si) : update_general(t, border ? 2 : 0, g, c, si, 5);
   }
+}
+
+template <bool COUNTER, bool FAST, bool OPEN, int MB>
+__global__ void __launch_bounds__(256) k_sweep2d_half(sweep_args p, int color) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= p.L * (p.L >> 1)) return;
+  half2d_site<COUNTER, FAST, OPEN, MB>(p, color, k, blockIdx.y * MB);
 }
 
 // ---------------------------------------------------------------- measure
@@ -779,14 +788,14 @@ __global__ void __launch_bounds__(256) k_measure(const uint64_t *__restrict__ s_
     atomicAdd(&cnt[(int64_t)sys0 * 128 + q], (&red[0][0][0])[q]);
 }
 
-__global__ void k_fill_zero(int *p, int n) {
+static __global__ void k_fill_zero(int *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
 
 // ---------------------------------------------------------------- masked swap
 // a ^= (a^b) & m, b ^= (a^b) & m (src/device.cuh:18-23)
-__global__ void __launch_bounds__(256) k_swap(uint64_t *a, uint64_t *b, uint64_t mask, int64_t N) {
+static __global__ void __launch_bounds__(256) k_swap(uint64_t *a, uint64_t *b, uint64_t mask, int64_t N) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
     const uint64_t va = a[i], vb = b[i];

@@ -9,15 +9,17 @@
 #include <string>
 #include <vector>
 
-#include "sa_hip.h"
-#include "sa_kernels.hpp"
-#include "sa/rng.hpp"
+#include "sa_internal.hpp"
 
 namespace {
 
 thread_local std::string g_last_error;
 
-int fail(int status, const char *fmt, ...) {
+constexpr int64_t kReplayWordsCap = int64_t(1) << 24;  // 128 MiB of staged random words
+
+}  // namespace
+
+int sa_fail(int status, const char *fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -26,40 +28,6 @@ int fail(int status, const char *fmt, ...) {
   g_last_error = buf;
   return status;
 }
-
-#define SA_HIP_TRY(expr)                                                                  \
-  do {                                                                                    \
-    hipError_t e_ = (expr);                                                               \
-    if (e_ != hipSuccess) return fail(SA_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
-
-constexpr int64_t kReplayWordsCap = int64_t(1) << 24;  // 128 MiB of staged random words
-
-}  // namespace
-
-struct sa_ladder {
-  int kind = 0, L = 0, nsys = 0, dim = 0;
-  int64_t N = 0;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  uint64_t *s = nullptr, *jx = nullptr, *jy = nullptr, *jz = nullptr;
-  uint64_t *s_alt = nullptr;  // second spin array of the fused 3D sweep (ping-pong)
-  sa::sys_table *tab = nullptr;
-  int *cnt = nullptr;
-  std::vector<sa::sys_table> htab;
-  std::vector<char> fast;  // per system: symmetric tables, low levels always accept
-  int rng_mode = SA_RNG_REPLAY;
-  uint64_t seed = 0, t = 0;
-  sa::mt19937_64 eng{};
-  uint64_t *words_d = nullptr, *words_h = nullptr;
-  int64_t words_cap = 0;  // sweeps that fit in the staging buffers
-  int *cnt_h = nullptr;
-  int *sys_list = nullptr;  // device copy of a measured subset
-  int sys_per_thread = 0;  // SA_TUNE_SYS_PER_THREAD, 0 = automatic
-  int sweep3d_kernel = 0;  // SA_TUNE_SWEEP3D_KERNEL, 0 = automatic
-  int rows_per_tile = 0;   // SA_TUNE_ROWS_PER_TILE, 0 = automatic
-  int xcd_map = -1;        // SA_TUNE_XCD_MAP, -1 = automatic
-};
 
 namespace {
 
@@ -81,8 +49,8 @@ void free_all(sa_ladder *l) {
 }
 
 int check_sys(const sa_ladder *l, int sys) {
-  if (!l) return fail(SA_ERR_ARG, "null ladder");
-  if (sys < 0 || sys >= l->nsys) return fail(SA_ERR_ARG, "system index %d out of range [0,%d)", sys, l->nsys);
+  if (!l) return sa_fail(SA_ERR_ARG, "null ladder");
+  if (sys < 0 || sys >= l->nsys) return sa_fail(SA_ERR_ARG, "system index %d out of range [0,%d)", sys, l->nsys);
   return SA_OK;
 }
 
@@ -93,9 +61,9 @@ int ensure_words(sa_ladder *l, int64_t sweeps) {
   l->words_d = nullptr;
   l->words_h = nullptr;
   l->words_cap = 0;
-  if (hipMalloc(&l->words_d, sweeps * l->N * 8) != hipSuccess) return fail(SA_ERR_NOMEM, "replay buffer");
+  if (hipMalloc(&l->words_d, sweeps * l->N * 8) != hipSuccess) return sa_fail(SA_ERR_NOMEM, "replay buffer");
   if (hipHostMalloc(&l->words_h, sweeps * l->N * 8, hipHostMallocDefault) != hipSuccess)
-    return fail(SA_ERR_NOMEM, "pinned replay buffer");
+    return sa_fail(SA_ERR_NOMEM, "pinned replay buffer");
   l->words_cap = sweeps;
   return SA_OK;
 }
@@ -268,13 +236,13 @@ const char *sa_status_string(int status) {
 int sa_version(void) { return 1; }
 
 int sa_ladder_create(int kind, int L, int nsys, sa_ladder **out) {
-  if (!out) return fail(SA_ERR_ARG, "null output handle");
+  if (!out) return sa_fail(SA_ERR_ARG, "null output handle");
   *out = nullptr;
-  if (kind < SA_KIND_2P || kind > SA_KIND_3D) return fail(SA_ERR_ARG, "unknown kind %d", kind);
-  if (nsys < 1) return fail(SA_ERR_ARG, "nsys must be >= 1");
-  if (L < 4 || (L & 1)) return fail(SA_ERR_UNSUPPORTED, "device path needs an even L >= 4 (got %d)", L);
+  if (kind < SA_KIND_2P || kind > SA_KIND_3D) return sa_fail(SA_ERR_ARG, "unknown kind %d", kind);
+  if (nsys < 1) return sa_fail(SA_ERR_ARG, "nsys must be >= 1");
+  if (L < 4 || (L & 1)) return sa_fail(SA_ERR_UNSUPPORTED, "device path needs an even L >= 4 (got %d)", L);
   if ((kind == SA_KIND_3D && L > 1024) || (kind != SA_KIND_3D && L > 32768))
-    return fail(SA_ERR_ARG, "L=%d too large", L);
+    return sa_fail(SA_ERR_ARG, "L=%d too large", L);
   sa_ladder *l = new sa_ladder();
   l->kind = kind;
   l->L = L;
@@ -285,8 +253,8 @@ int sa_ladder_create(int kind, int L, int nsys, sa_ladder **out) {
   int rc = SA_OK;
   auto bad = [&](hipError_t e, const char *what) {
     if (e == hipSuccess) return false;
-    rc = (e == hipErrorOutOfMemory) ? fail(SA_ERR_NOMEM, "%s: %s", what, hipGetErrorString(e))
-                                    : fail(SA_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    rc = (e == hipErrorOutOfMemory) ? sa_fail(SA_ERR_NOMEM, "%s: %s", what, hipGetErrorString(e))
+                                    : sa_fail(SA_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
     return true;
   };
   if (bad(hipGetDevice(&l->device), "hipGetDevice") || bad(hipStreamCreate(&l->stream), "hipStreamCreate") ||
@@ -332,14 +300,14 @@ int sa_ladder_nsys(const sa_ladder *l) { return l ? l->nsys : 0; }
 void *sa_ladder_stream(sa_ladder *l) { return l ? (void *)l->stream : nullptr; }
 
 int sa_ladder_set_rng(sa_ladder *l, int mode) {
-  if (!l) return fail(SA_ERR_ARG, "null ladder");
-  if (mode != SA_RNG_REPLAY && mode != SA_RNG_COUNTER) return fail(SA_ERR_ARG, "unknown rng mode %d", mode);
+  if (!l) return sa_fail(SA_ERR_ARG, "null ladder");
+  if (mode != SA_RNG_REPLAY && mode != SA_RNG_COUNTER) return sa_fail(SA_ERR_ARG, "unknown rng mode %d", mode);
   l->rng_mode = mode;
   return SA_OK;
 }
 
 int sa_ladder_seed(sa_ladder *l, uint64_t seed) {
-  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  if (!l) return sa_fail(SA_ERR_ARG, "null ladder");
   l->eng.seed(seed);
   l->seed = seed;
   l->t = 0;
@@ -347,34 +315,34 @@ int sa_ladder_seed(sa_ladder *l, uint64_t seed) {
 }
 
 int sa_ladder_get_sweep_index(const sa_ladder *l, uint64_t *t) {
-  if (!l || !t) return fail(SA_ERR_ARG, "null argument");
+  if (!l || !t) return sa_fail(SA_ERR_ARG, "null argument");
   *t = l->t;
   return SA_OK;
 }
 
 int sa_ladder_set_sweep_index(sa_ladder *l, uint64_t t) {
-  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  if (!l) return sa_fail(SA_ERR_ARG, "null ladder");
   l->t = t;
   return SA_OK;
 }
 
 int sa_ladder_get_engine(const sa_ladder *l, uint64_t state[313]) {
-  if (!l || !state) return fail(SA_ERR_ARG, "null argument");
+  if (!l || !state) return sa_fail(SA_ERR_ARG, "null argument");
   std::memcpy(state, l->eng.x, 312 * 8);
   state[312] = l->eng.p;
   return SA_OK;
 }
 
 int sa_ladder_set_engine(sa_ladder *l, const uint64_t state[313]) {
-  if (!l || !state) return fail(SA_ERR_ARG, "null argument");
-  if (state[312] > 312) return fail(SA_ERR_ARG, "engine position %llu > 312", (unsigned long long)state[312]);
+  if (!l || !state) return sa_fail(SA_ERR_ARG, "null argument");
+  if (state[312] > 312) return sa_fail(SA_ERR_ARG, "engine position %llu > 312", (unsigned long long)state[312]);
   std::memcpy(l->eng.x, state, 312 * 8);
   l->eng.p = state[312];
   return SA_OK;
 }
 
 int sa_ladder_set_couplings(sa_ladder *l, const uint64_t *jx, const uint64_t *jy, const uint64_t *jz) {
-  if (!l || !jx || !jy || (l->kind == SA_KIND_3D && !jz)) return fail(SA_ERR_ARG, "null argument");
+  if (!l || !jx || !jy || (l->kind == SA_KIND_3D && !jz)) return sa_fail(SA_ERR_ARG, "null argument");
   const size_t bytes = l->N * 8;
   SA_HIP_TRY(hipMemcpyAsync(l->jx, jx, bytes, hipMemcpyHostToDevice, l->stream));
   SA_HIP_TRY(hipMemcpyAsync(l->jy, jy, bytes, hipMemcpyHostToDevice, l->stream));
@@ -384,7 +352,7 @@ int sa_ladder_set_couplings(sa_ladder *l, const uint64_t *jx, const uint64_t *jy
 }
 
 int sa_ladder_get_couplings(const sa_ladder *l, uint64_t *jx, uint64_t *jy, uint64_t *jz) {
-  if (!l || !jx || !jy || (l->kind == SA_KIND_3D && !jz)) return fail(SA_ERR_ARG, "null argument");
+  if (!l || !jx || !jy || (l->kind == SA_KIND_3D && !jz)) return sa_fail(SA_ERR_ARG, "null argument");
   const size_t bytes = l->N * 8;
   SA_HIP_TRY(hipMemcpyAsync(jx, l->jx, bytes, hipMemcpyDeviceToHost, l->stream));
   SA_HIP_TRY(hipMemcpyAsync(jy, l->jy, bytes, hipMemcpyDeviceToHost, l->stream));
@@ -394,7 +362,7 @@ int sa_ladder_get_couplings(const sa_ladder *l, uint64_t *jx, uint64_t *jy, uint
 }
 
 int sa_ladder_init_couplings(sa_ladder *l) {
-  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  if (!l) return sa_fail(SA_ERR_ARG, "null ladder");
   const int64_t N = l->N;
   const int d = bonds_per_site(l->kind);
   std::vector<uint64_t> jx(N), jy(N), jz(l->kind == SA_KIND_3D ? N : 1);
@@ -408,7 +376,7 @@ int sa_ladder_init_couplings(sa_ladder *l) {
 }
 
 int sa_ladder_init_spins(sa_ladder *l) {
-  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  if (!l) return sa_fail(SA_ERR_ARG, "null ladder");
   std::vector<uint64_t> s(l->N);
   l->eng.fill(s.data(), l->N);
   for (int i = 0; i < l->nsys; ++i)
@@ -420,7 +388,7 @@ int sa_ladder_init_spins(sa_ladder *l) {
 int sa_ladder_set_spins(sa_ladder *l, int sys, const uint64_t *s) {
   int rc = check_sys(l, sys);
   if (rc) return rc;
-  if (!s) return fail(SA_ERR_ARG, "null spins");
+  if (!s) return sa_fail(SA_ERR_ARG, "null spins");
   SA_HIP_TRY(hipMemcpyAsync(l->s + (int64_t)sys * l->N, s, l->N * 8, hipMemcpyHostToDevice, l->stream));
   SA_HIP_TRY(hipStreamSynchronize(l->stream));
   return SA_OK;
@@ -429,19 +397,19 @@ int sa_ladder_set_spins(sa_ladder *l, int sys, const uint64_t *s) {
 int sa_ladder_get_spins(const sa_ladder *l, int sys, uint64_t *s) {
   int rc = check_sys(l, sys);
   if (rc) return rc;
-  if (!s) return fail(SA_ERR_ARG, "null spins");
+  if (!s) return sa_fail(SA_ERR_ARG, "null spins");
   SA_HIP_TRY(hipMemcpyAsync(s, l->s + (int64_t)sys * l->N, l->N * 8, hipMemcpyDeviceToHost, l->stream));
   SA_HIP_TRY(hipStreamSynchronize(l->stream));
   return SA_OK;
 }
 
-int sa_ladder_set_temperature(sa_ladder *l, int sys, double T, double h, double *boltz_out) {
-  int rc = check_sys(l, sys);
-  if (rc) return rc;
+}  // extern "C"
+
+void sa_detail::make_table(int kind, double T, double h, sa::sys_table &t, bool &fast, double *boltz_out) {
   double b[20];
-  sa::sys_table t{};
-  const int nb = l->kind == SA_KIND_3D ? 6 : 4;
-  if (l->kind == SA_KIND_3D) {
+  t = sa::sys_table{};
+  const int nb = kind == SA_KIND_3D ? 6 : 4;
+  if (kind == SA_KIND_3D) {
     sa::boltz_3d(T, h, b);
     fill_table(t, 0, b, 7);
     fill_table(t, 1, b + 7, 7);
@@ -452,28 +420,38 @@ int sa_ladder_set_temperature(sa_ladder *l, int sys, double T, double h, double 
     fill_table(t, 2, b + 10, 5);
     fill_table(t, 3, b + 15, 5);
   }
-  if (boltz_out) std::memcpy(boltz_out, b, sizeof(double) * (l->kind == SA_KIND_3D ? 14 : 20));
+  if (boltz_out) std::memcpy(boltz_out, b, sizeof(double) * (kind == SA_KIND_3D ? 14 : 20));
   // fast kernels: identical spin tables, levels 0..nb/2 always accepted
-  bool fast = l->kind != SA_KIND_2P && std::memcmp(t.thr[0], t.thr[1], sizeof t.thr[0]) == 0 &&
-              t.always[0] == t.always[1];
+  fast = kind != SA_KIND_2P && std::memcmp(t.thr[0], t.thr[1], sizeof t.thr[0]) == 0 && t.always[0] == t.always[1];
   const uint32_t low = (1u << (nb / 2 + 1)) - 1u;
   fast = fast && (t.always[0] & low) == low && (t.always[0] >> (nb / 2 + 1)) == 0;
+}
+
+extern "C" {
+
+int sa_ladder_set_temperature(sa_ladder *l, int sys, double T, double h, double *boltz_out) {
+  int rc = check_sys(l, sys);
+  if (rc) return rc;
+  sa::sys_table t;
+  bool fast = false;
+  sa_detail::make_table(l->kind, T, h, t, fast, boltz_out);
   l->htab[sys] = t;
   l->fast[sys] = fast;
+  // stream-ordered: a pageable-source copy is staged before the call returns, so no
+  // wait is needed; every later launch on the stream sees the new table
   SA_HIP_TRY(hipMemcpyAsync(l->tab + sys, &l->htab[sys], sizeof(sa::sys_table), hipMemcpyHostToDevice, l->stream));
-  SA_HIP_TRY(hipStreamSynchronize(l->stream));
   return SA_OK;
 }
 
 int sa_ladder_sweep(sa_ladder *l, int nsweeps) {
-  if (!l) return fail(SA_ERR_ARG, "null ladder");
-  if (nsweeps < 0) return fail(SA_ERR_ARG, "negative sweep count");
+  if (!l) return sa_fail(SA_ERR_ARG, "null ladder");
+  if (nsweeps < 0) return sa_fail(SA_ERR_ARG, "negative sweep count");
   if (nsweeps == 0) return SA_OK;
   if (use_fused(l) && !l->s_alt) {
     const hipError_t e = hipMalloc(&l->s_alt, (size_t)l->nsys * l->N * 8);
     if (e != hipSuccess) {
       l->s_alt = nullptr;
-      return fail(SA_ERR_NOMEM, "second spin array: %s", hipGetErrorString(e));
+      return sa_fail(SA_ERR_NOMEM, "second spin array: %s", hipGetErrorString(e));
     }
   }
   if (l->rng_mode == SA_RNG_COUNTER) {
@@ -540,8 +518,21 @@ void counts_to_EM(const sa_ladder *l, int j, int32_t *E, int32_t *M) {
 
 }  // namespace
 
+int sa_ladder_anneal(sa_ladder *l, const double *T, int nsweeps, double h) {
+  if (!l || (!T && nsweeps > 0) || nsweeps < 0) return sa_fail(SA_ERR_ARG, "bad argument");
+  for (int t = 0; t < nsweeps; ++t) {
+    for (int j = 0; j < l->nsys; ++j) {
+      const int rc = sa_ladder_set_temperature(l, j, T[(int64_t)t * l->nsys + j], h, nullptr);
+      if (rc) return rc;
+    }
+    const int rc = sa_ladder_sweep(l, 1);
+    if (rc) return rc;
+  }
+  return SA_OK;
+}
+
 int sa_ladder_measure(sa_ladder *l, int32_t *E, int32_t *M) {
-  if (!l || !E || !M) return fail(SA_ERR_ARG, "null argument");
+  if (!l || !E || !M) return sa_fail(SA_ERR_ARG, "null argument");
   const int rc = measure_list(l, nullptr, l->nsys);
   if (rc) return rc;
   for (int i = 0; i < l->nsys; ++i) counts_to_EM(l, i, E + i * 64, M + i * 64);
@@ -549,10 +540,10 @@ int sa_ladder_measure(sa_ladder *l, int32_t *E, int32_t *M) {
 }
 
 int sa_ladder_measure_systems(sa_ladder *l, const int *sys, int n, int32_t *E, int32_t *M) {
-  if (!l || !sys || !E || !M) return fail(SA_ERR_ARG, "null argument");
-  if (n < 1 || n > l->nsys) return fail(SA_ERR_ARG, "measure of %d systems out of %d", n, l->nsys);
+  if (!l || !sys || !E || !M) return sa_fail(SA_ERR_ARG, "null argument");
+  if (n < 1 || n > l->nsys) return sa_fail(SA_ERR_ARG, "measure of %d systems out of %d", n, l->nsys);
   for (int j = 0; j < n; ++j)
-    if (sys[j] < 0 || sys[j] >= l->nsys) return fail(SA_ERR_ARG, "system index %d out of range", sys[j]);
+    if (sys[j] < 0 || sys[j] >= l->nsys) return sa_fail(SA_ERR_ARG, "system index %d out of range", sys[j]);
   const int rc = measure_list(l, sys, n);
   if (rc) return rc;
   for (int j = 0; j < n; ++j) counts_to_EM(l, j, E + j * 64, M + j * 64);
@@ -573,35 +564,35 @@ int sa_ladder_swap(sa_ladder *l, int a, int b, uint64_t mask) {
 }
 
 int sa_ladder_set_tuning(sa_ladder *l, int key, int value) {
-  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  if (!l) return sa_fail(SA_ERR_ARG, "null ladder");
   switch (key) {
     case SA_TUNE_SYS_PER_THREAD:
       if (value != 0 && value != 1 && value != 2 && value != 4)
-        return fail(SA_ERR_ARG, "systems per thread must be 0 (auto), 1, 2 or 4");
+        return sa_fail(SA_ERR_ARG, "systems per thread must be 0 (auto), 1, 2 or 4");
       if (value > 0 && l->nsys % value != 0)
-        return fail(SA_ERR_ARG, "%d systems per thread do not divide nsys=%d", value, l->nsys);
+        return sa_fail(SA_ERR_ARG, "%d systems per thread do not divide nsys=%d", value, l->nsys);
       l->sys_per_thread = value;
       return SA_OK;
     case SA_TUNE_SWEEP3D_KERNEL:
-      if (value < 0 || value > 2) return fail(SA_ERR_ARG, "3D sweep kernel must be 0 (auto), 1 or 2");
+      if (value < 0 || value > 2) return sa_fail(SA_ERR_ARG, "3D sweep kernel must be 0 (auto), 1 or 2");
       l->sweep3d_kernel = value;
       return SA_OK;
     case SA_TUNE_ROWS_PER_TILE:
       if (value != 0 && value != 4 && value != 8 && value != 16)
-        return fail(SA_ERR_ARG, "rows per tile must be 0 (auto), 4, 8 or 16");
+        return sa_fail(SA_ERR_ARG, "rows per tile must be 0 (auto), 4, 8 or 16");
       l->rows_per_tile = value;
       return SA_OK;
     case SA_TUNE_XCD_MAP:
-      if (value < -1 || value > 1) return fail(SA_ERR_ARG, "XCD map must be -1 (auto), 0 or 1");
+      if (value < -1 || value > 1) return sa_fail(SA_ERR_ARG, "XCD map must be -1 (auto), 0 or 1");
       l->xcd_map = value;
       return SA_OK;
     default:
-      return fail(SA_ERR_ARG, "unknown tuning key %d", key);
+      return sa_fail(SA_ERR_ARG, "unknown tuning key %d", key);
   }
 }
 
 int sa_ladder_get_tuning(const sa_ladder *l, int key, int *value) {
-  if (!l || !value) return fail(SA_ERR_ARG, "null argument");
+  if (!l || !value) return sa_fail(SA_ERR_ARG, "null argument");
   const bool fused = use_fused(l);
   const fused_cfg c = fused ? pick_fused(l) : fused_cfg{0, 0, 0};
   switch (key) {
@@ -609,12 +600,12 @@ int sa_ladder_get_tuning(const sa_ladder *l, int key, int *value) {
     case SA_TUNE_SWEEP3D_KERNEL: *value = l->kind != SA_KIND_3D ? 0 : (fused ? 2 : 1); return SA_OK;
     case SA_TUNE_ROWS_PER_TILE: *value = c.th; return SA_OK;
     case SA_TUNE_XCD_MAP: *value = c.xcd; return SA_OK;
-    default: return fail(SA_ERR_ARG, "unknown tuning key %d", key);
+    default: return sa_fail(SA_ERR_ARG, "unknown tuning key %d", key);
   }
 }
 
 int sa_ladder_sync(sa_ladder *l) {
-  if (!l) return fail(SA_ERR_ARG, "null ladder");
+  if (!l) return sa_fail(SA_ERR_ARG, "null ladder");
   SA_HIP_TRY(hipStreamSynchronize(l->stream));
   return SA_OK;
 }
