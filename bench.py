@@ -91,12 +91,60 @@ def cpu_baseline():
     return dict(value=attempts / secs / 1e9, unit="attempts/ns", cores=cores, kind=kind, sample=sample)
 
 
+def batched_vs_plain(sa, n=512):
+    """The batched form against the plain path on the same instances.
+
+    Workload: n independent realisations of uneven size (60% 2D periodic with L in
+    {32,48,64,96,128}, 40% 3D with L in {16,24,32,48}), each its own linear anneal
+    3.0 -> 0.1 over 500, 1000 or 2000 sweeps, counter RNG.  Batched: one Batch
+    (one launch per half-sweep for all unfinished instances, hipGraph chunks, block
+    size and chunk length tuned online inside the timed region).  Plain: a ladder
+    per instance driven by sa_ladder_anneal (temperature upload + sweep launch per
+    sweep, each instance on its own stream).  Set-up (seeding, couplings) is outside
+    both timed regions.  Every instance's final spins are compared bit for bit."""
+    from simulated_annealing_amd import workloads
+
+    W = workloads.uneven_instances(n)
+    attempts = sum(w.sites * 64 * w.nsweeps for w in W)
+    b = sa.Batch()
+    ids = [b.add(w.kind, w.L, w.seed, w.nsweeps, schedule=workloads.anneal_schedule(w.nsweeps)) for w in W]
+    t0 = time.perf_counter()
+    assert b.run() == 0
+    dt_b = time.perf_counter() - t0
+    lads = []
+    for w in W:
+        lad = sa.Ladder(w.kind, w.L, 1)
+        lad.set_rng(sa.RNG_COUNTER)
+        lad.seed_like_main(w.seed)
+        lads.append(lad)
+    for lad in lads:
+        lad.sync()
+    t0 = time.perf_counter()
+    for lad, w in zip(lads, W):
+        lad.anneal(workloads.anneal_schedule(w.nsweeps))
+    for lad in lads:
+        lad.sync()
+    dt_p = time.perf_counter() - t0
+    same = sum(int((b.spins(i) == lad.spins(0)).all()) for i, lad in zip(ids, lads))
+    out = dict(workload=f"{n} uneven instances: 60% 2D L in {{32..128}}, 40% 3D L in {{16..48}}, "
+                        "linear anneal 3.0->0.1 over 500/1000/2000 sweeps",
+               attempts=attempts, batched_attempts_per_ns=round(attempts / dt_b / 1e9, 1),
+               plain_attempts_per_ns=round(attempts / dt_p / 1e9, 1), speedup=round(dt_p / dt_b, 2),
+               batched_s=round(dt_b, 4), plain_s=round(dt_p, 4), bitexact_instances=f"{same}/{n}",
+               block=b.tuning(sa.TUNE_BATCH_BLOCK), chunk=b.tuning(sa.TUNE_BATCH_CHUNK))
+    for lad in lads:
+        lad.close()
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the masked-swap kernel on two 128 MiB systems (known traffic: 256 MiB read, "
                          "256 MiB written per launch) to scale the FETCH_SIZE/WRITE_SIZE counters")
@@ -160,9 +208,11 @@ def main():
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                     kernel=roofline_kernel(fused), launch_us=round(launch_us, 2), alg_bytes_per_launch=alg_bytes)
+    pt.close()
+    batched = None if args.no_batch else batched_vs_plain(sa)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
-        "metric": "replica spin-flip attempts/ns (1 GPU, 3D L=128)",
+        "metric": "replica spin-flip attempts/ns (1 GPU, 3D L=128); batched vs plain speedup",
         "value": round(value, 1),
         "unit": "attempts/ns",
         "n_gpus": 1,
@@ -180,9 +230,9 @@ def main():
                    "sweeps_timed": args.steps * SWEEPS},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched_vs_plain": batched,
     }
     print(json.dumps(line), flush=True)
-    pt.close()
 
 
 if __name__ == "__main__":

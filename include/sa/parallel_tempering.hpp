@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "sa/annealing.hpp"
+#include "sa/exchange.hpp"
 #include "sa/hip_sys.hpp"
 #include "sa/rng.hpp"
 
@@ -245,12 +246,9 @@ class parallel_tempering {
     for (size_t j = 0; j < (words > 0 ? words : 1); ++j) mask[j] = 0ULL;
     int a = 0;
     const double dB = (1. / T_[i] - 1. / T_[i + 1]);
-    for (size_t j = 0; j < E_[i].size(); ++j) {
-      const float dE = -(E_[i][j] - E_[i + 1][j]);
-      if (((dE * dB) > 0.) || rand < std::exp(dB * dE)) {
-        mask[j / 64] |= 1ULL << (j % 64);
-        a++;
-      }
+    for (size_t j = 0; j < E_[i].size(); j += 64) {
+      const int n = (int)std::min<size_t>(64, E_[i].size() - j);
+      mask[j / 64] = exchange_word(E_[i].data() + j, E_[i + 1].data() + j, dB, rand, n, a);
     }
     num_switch_[i](a / (double)64.);
     return mask;

@@ -199,12 +199,30 @@ int sa_batch_destroy(sa_batch *b);
  * *id receives the instance id. */
 int sa_batch_add(sa_batch *b, int kind, int L, int nsys, uint64_t seed, int nsweeps, const double *T, int per_sweep,
                  double h, int *id);
+/* Add a parallel-tempering instance: nT systems at the temperatures T (sorted
+ * ascending, as parallel_tempering does), field h, `nsteps` steps of `every` sweeps
+ * each followed by one exchange attempt between a random adjacent pair drawn from
+ * the instance's own mt19937(123) (parallel_tempering::step(0),
+ * src/parallel_tempering.hpp:81-115; same pair sequence, masks and spins as
+ * sa_pt_run with the counter RNG). */
+int sa_batch_add_pt(sa_batch *b, int kind, int L, int nT, const double *T, double h, uint64_t seed, int every,
+                    int nsteps, int *id);
 /* Run up to `sweeps` sweeps of every unfinished instance (sweeps <= 0: until all
- * are done).  Returns the number of unfinished instances (>= 0) or an sa_status. */
+ * are done); chunks stop at every exchange point.  Returns the number of
+ * unfinished instances (>= 0) or an sa_status. */
 int sa_batch_run(sa_batch *b, int sweeps);
 int sa_batch_sweeps_done(sa_batch *b, int id, int *t);
 int sa_batch_get_spins(sa_batch *b, int id, int sys, uint64_t *s);
 int sa_batch_measure(sa_batch *b, int id, int32_t *E, int32_t *M);
+/* A parallel-tempering instance's exchange log so far: pairs/us/masks (capacity
+ * entries each, any may be NULL), E_min[64] (per lane the largest measure() value of
+ * the coldest system over the steps, parallel_tempering::min_E_update) and
+ * acceptance[nT-1] (mean accepted lane fraction per pair).  Returns the number of
+ * steps logged (>= 0) or an sa_status. */
+int sa_batch_pt_result(sa_batch *b, int id, int64_t capacity, int32_t *pairs, double *us, uint64_t *masks,
+                       float *E_min, double *acceptance);
+/* device bytes held by the batch's live instances */
+int64_t sa_batch_bytes(const sa_batch *b);
 /* the instance's ladder (owned by the batch; valid until sa_batch_remove) */
 sa_ladder *sa_batch_ladder(sa_batch *b, int id);
 /* free the instance's device memory; its id is not reused */
@@ -216,6 +234,71 @@ void *sa_batch_stream(sa_batch *b);
 enum sa_batch_tune_key { SA_TUNE_BATCH_BLOCK = 101, SA_TUNE_BATCH_CHUNK = 102 };
 int sa_batch_set_tuning(sa_batch *b, int key, int value);
 int sa_batch_get_tuning(sa_batch *b, int key, int *value);
+
+/* ---- serving loop (no counterpart in the reference) --------------------------
+ * Continuous batching of anneal / parallel-tempering requests over one sa_batch:
+ * requests arrive on the server clock, are admitted in arrival order as device
+ * memory (hbm_budget) and slots (max_active) free up, join the next chunk of
+ * sweeps, and are retired with a digest of their final spins and their energies.
+ * Each request's results are those of running it alone (sa_ladder_anneal on a
+ * ladder seeded like main, or sa_pt_run with the counter RNG). */
+enum sa_request_mode { SA_REQ_ANNEAL = 0, SA_REQ_PT = 1 };
+typedef struct sa_request {
+  int kind, L;
+  int mode;      /* SA_REQ_ANNEAL or SA_REQ_PT */
+  int nT;        /* PT: temperatures (>= 2); ignored for an anneal */
+  int nsweeps;   /* total sweeps; PT: a multiple of `every` */
+  int every;     /* PT: sweeps between exchange attempts */
+  double T0, T1; /* anneal: linear T0 -> T1 over the sweeps; PT: geometric ladder T0 .. T1 */
+  double h;
+  uint64_t seed; /* set_seed, init_J, set_seed, init_rand (src/main_gpu.cu:262-276) */
+  double arrival; /* seconds on the server clock; < 0: the time of submission */
+} sa_request;
+typedef struct sa_request_result {
+  int64_t id;
+  int status; /* SA_OK, an sa_status for a refused request, SA_ERR_STATE while unfinished */
+  int steps;  /* PT exchange attempts logged */
+  double arrival, admitted, finished; /* server clock, seconds */
+  int64_t bytes;   /* device memory held while admitted */
+  double attempts; /* spin-flip attempts: sites x 64 x systems x sweeps */
+  uint64_t spins_digest; /* sa_digest over all systems' spin words (system j from index j*N) */
+  uint64_t log_digest;   /* PT: chained splitmix64 of (pair, uniform bits, mask) per step */
+  int32_t E[64], M[64];  /* final integer energy / magnetisation of system 0 per lane */
+  float E_min[64];       /* PT: largest coldest-system measure() value over the steps */
+} sa_request_result;
+typedef struct sa_serve_config {
+  int64_t hbm_budget; /* device bytes for admitted requests (0: half the free memory) */
+  int max_active;     /* requests on the device at once (0: 512) */
+  int chunk;          /* sweeps between admission points (0: 10) */
+  int digests;        /* 1: read final spins back and digest them */
+} sa_serve_config;
+typedef struct sa_serve_stats {
+  int64_t submitted, finished, pending;
+  int active, peak_active;
+  int64_t bytes, peak_bytes, hbm_budget;
+  double busy_s, clock_s;
+} sa_serve_stats;
+typedef struct sa_serve sa_serve;
+int sa_serve_create(const sa_serve_config *cfg, sa_serve **out);
+int sa_serve_destroy(sa_serve *s);
+/* seconds since the server was created (the clock arrivals are given on) */
+double sa_serve_clock(const sa_serve *s);
+/* thread-safe; *id = request id (results are indexed by it) */
+int sa_serve_submit(sa_serve *s, const sa_request *r, int64_t *id);
+/* run the loop until n_finished requests have finished in total (<= 0: until every
+ * submitted request has) */
+int sa_serve_run(sa_serve *s, int64_t n_finished);
+/* results of requests first .. first+capacity-1; returns how many were copied */
+int64_t sa_serve_results(sa_serve *s, int64_t first, int64_t capacity, sa_request_result *out);
+int sa_serve_get_stats(sa_serve *s, sa_serve_stats *st);
+/* the temperatures a request runs at (nsweeps for an anneal, nT for PT); returns the
+ * count, T may be NULL */
+int sa_request_temperatures(const sa_request *r, double *T, int64_t capacity);
+/* device bytes a request holds while admitted (what admission charges against
+ * hbm_budget), or an sa_status for an invalid request */
+int64_t sa_request_bytes(const sa_request *r);
+/* XOR over i of splitmix64(w[i] + (first_index + i) * 0x9E3779B97F4A7C15) */
+uint64_t sa_digest(const uint64_t *w, int64_t n, uint64_t first_index);
 
 #ifdef __cplusplus
 }

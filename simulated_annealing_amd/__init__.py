@@ -35,10 +35,37 @@ EXPORTS = [
     "sa_pt_run", "sa_anneal_run",
     "sa_pt_create", "sa_pt_step", "sa_pt_sync", "sa_pt_steps_done", "sa_pt_result", "sa_pt_ladder",
     "sa_pt_destroy",
-    "sa_batch_create", "sa_batch_destroy", "sa_batch_add", "sa_batch_run", "sa_batch_sweeps_done",
-    "sa_batch_get_spins", "sa_batch_measure", "sa_batch_ladder", "sa_batch_remove", "sa_batch_stream",
-    "sa_batch_set_tuning", "sa_batch_get_tuning",
+    "sa_batch_create", "sa_batch_destroy", "sa_batch_add", "sa_batch_add_pt", "sa_batch_run",
+    "sa_batch_sweeps_done", "sa_batch_get_spins", "sa_batch_measure", "sa_batch_pt_result", "sa_batch_bytes",
+    "sa_batch_ladder", "sa_batch_remove", "sa_batch_stream", "sa_batch_set_tuning", "sa_batch_get_tuning",
+    "sa_serve_create", "sa_serve_destroy", "sa_serve_clock", "sa_serve_submit", "sa_serve_run", "sa_serve_results",
+    "sa_serve_get_stats", "sa_request_temperatures", "sa_request_bytes", "sa_digest",
 ]
+
+REQ_ANNEAL, REQ_PT = 0, 1
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("L", C.c_int), ("mode", C.c_int), ("nT", C.c_int), ("nsweeps", C.c_int),
+                ("every", C.c_int), ("T0", C.c_double), ("T1", C.c_double), ("h", C.c_double),
+                ("seed", C.c_uint64), ("arrival", C.c_double)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int), ("steps", C.c_int), ("arrival", C.c_double),
+                ("admitted", C.c_double), ("finished", C.c_double), ("bytes", C.c_int64),
+                ("attempts", C.c_double), ("spins_digest", C.c_uint64), ("log_digest", C.c_uint64),
+                ("E", C.c_int32 * 64), ("M", C.c_int32 * 64), ("E_min", C.c_float * 64)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("max_active", C.c_int), ("chunk", C.c_int), ("digests", C.c_int)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("finished", C.c_int64), ("pending", C.c_int64), ("active", C.c_int),
+                ("peak_active", C.c_int), ("bytes", C.c_int64), ("peak_bytes", C.c_int64),
+                ("hbm_budget", C.c_int64), ("busy_s", C.c_double), ("clock_s", C.c_double)]
 
 
 class SAError(RuntimeError):
@@ -110,6 +137,11 @@ def lib():
         L.sa_batch_destroy.argtypes = [_vp]
         L.sa_batch_add.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, _f64p, C.c_int, C.c_double,
                                    C.POINTER(C.c_int)]
+        L.sa_batch_add_pt.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _f64p, C.c_double, C.c_uint64, C.c_int,
+                                      C.c_int, C.POINTER(C.c_int)]
+        L.sa_batch_pt_result.argtypes = [_vp, C.c_int, C.c_int64, _vp, _vp, _vp, _vp, _vp]
+        L.sa_batch_bytes.argtypes = [_vp]
+        L.sa_batch_bytes.restype = C.c_int64
         L.sa_batch_run.argtypes = [_vp, C.c_int]
         L.sa_batch_sweeps_done.argtypes = [_vp, C.c_int, C.POINTER(C.c_int)]
         L.sa_batch_get_spins.argtypes = [_vp, C.c_int, C.c_int, _u64p]
@@ -119,6 +151,20 @@ def lib():
         L.sa_batch_remove.argtypes = [_vp, C.c_int]
         L.sa_batch_stream.argtypes = [_vp]
         L.sa_batch_stream.restype = _vp
+        L.sa_serve_create.argtypes = [C.POINTER(ServeConfig), C.POINTER(_vp)]
+        L.sa_serve_destroy.argtypes = [_vp]
+        L.sa_serve_clock.argtypes = [_vp]
+        L.sa_serve_clock.restype = C.c_double
+        L.sa_serve_submit.argtypes = [_vp, C.POINTER(Request), C.POINTER(C.c_int64)]
+        L.sa_serve_run.argtypes = [_vp, C.c_int64]
+        L.sa_serve_results.argtypes = [_vp, C.c_int64, C.c_int64, C.POINTER(RequestResult)]
+        L.sa_serve_results.restype = C.c_int64
+        L.sa_serve_get_stats.argtypes = [_vp, C.POINTER(ServeStats)]
+        L.sa_request_temperatures.argtypes = [C.POINTER(Request), _vp, C.c_int64]
+        L.sa_request_bytes.argtypes = [C.POINTER(Request)]
+        L.sa_request_bytes.restype = C.c_int64
+        L.sa_digest.argtypes = [_u64p, C.c_int64, C.c_uint64]
+        L.sa_digest.restype = C.c_uint64
         L.sa_batch_set_tuning.argtypes = [_vp, C.c_int, C.c_int]
         L.sa_batch_get_tuning.argtypes = [_vp, C.c_int, C.POINTER(C.c_int)]
         _lib = L
@@ -359,6 +405,29 @@ class Batch:
         self.info[i.value] = (kind, L, nsys)
         return i.value
 
+    def add_pt(self, kind, L, T, seed, every, nsteps, h=0.0) -> int:
+        """A parallel-tempering instance: len(T) systems, nsteps x (every sweeps + one exchange)."""
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        i = C.c_int()
+        check(lib().sa_batch_add_pt(self.b, kind, L, T.size, T, h, seed, every, nsteps, C.byref(i)), "sa_batch_add_pt")
+        self.info[i.value] = (kind, L, T.size)
+        return i.value
+
+    def pt_result(self, i: int):
+        nT = self.info[i][2]
+        n = check(lib().sa_batch_pt_result(self.b, i, 0, None, None, None, None, None), "sa_batch_pt_result")
+        pairs = np.zeros(max(n, 1), dtype=np.int32)
+        us = np.zeros(max(n, 1), dtype=np.float64)
+        masks = np.zeros(max(n, 1), dtype=np.uint64)
+        E_min = np.zeros(64, dtype=np.float32)
+        acc = np.zeros(nT - 1, dtype=np.float64)
+        check(lib().sa_batch_pt_result(self.b, i, n, _ptr(pairs), _ptr(us), _ptr(masks), _ptr(E_min), _ptr(acc)),
+              "sa_batch_pt_result")
+        return dict(pairs=pairs[:n], us=us[:n], masks=masks[:n], E_min=E_min, acceptance=acc)
+
+    def bytes(self) -> int:
+        return lib().sa_batch_bytes(self.b)
+
     def run(self, sweeps: int = 0) -> int:
         return check(lib().sa_batch_run(self.b, sweeps), "sa_batch_run")
 
@@ -399,6 +468,103 @@ class Batch:
         if getattr(self, "b", None):
             lib().sa_batch_destroy(self.b)
             self.b = None
+
+    def __del__(self):
+        self.close()
+
+
+def make_request(kind, L, mode, nsweeps, seed, nT=1, every=10, T0=None, T1=None, h=0.0, arrival=0.0) -> Request:
+    """An anneal (linear T0 -> T1, default 3.0 -> 0.1) or a PT run (geometric ladder, default 0.5 .. 2.0)."""
+    if T0 is None:
+        T0 = 3.0 if mode == REQ_ANNEAL else 0.5
+    if T1 is None:
+        T1 = 0.1 if mode == REQ_ANNEAL else 2.0
+    return Request(kind, L, mode, nT if mode == REQ_PT else 1, nsweeps, every, T0, T1, h, seed, arrival)
+
+
+def request_temperatures(r: Request) -> np.ndarray:
+    """The temperatures the server runs a request at: per sweep (anneal) or the ladder (PT)."""
+    n = check(lib().sa_request_temperatures(C.byref(r), None, 0), "sa_request_temperatures")
+    T = np.zeros(n, dtype=np.float64)
+    check(lib().sa_request_temperatures(C.byref(r), _ptr(T), n), "sa_request_temperatures")
+    return T
+
+
+def request_bytes(r: Request) -> int:
+    """Device bytes the request holds while admitted."""
+    return check(lib().sa_request_bytes(C.byref(r)), "sa_request_bytes")
+
+
+_GOLD = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _mix64(z: np.ndarray) -> np.ndarray:
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def digest(words, first_index: int = 0) -> int:
+    """XOR over i of splitmix64(w[i] + (first_index + i) * golden) -- sa_digest in numpy."""
+    w = np.ascontiguousarray(words, dtype=np.uint64).reshape(-1)
+    with np.errstate(over="ignore"):
+        idx = (np.arange(w.size, dtype=np.uint64) + np.uint64(first_index)) * _GOLD
+        return int(np.bitwise_xor.reduce(_mix64(w + idx))) if w.size else 0
+
+
+def log_digest(pairs, us, masks) -> int:
+    """Chained splitmix64 of (pair, uniform bits, mask) per exchange step."""
+    def mix(z):
+        z &= 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    h = 0
+    ub = np.ascontiguousarray(us, dtype=np.float64).view(np.uint64)
+    for p, u, m in zip(pairs, ub, masks):
+        h = mix(h ^ (int(p) & 0xFFFFFFFF))
+        h = mix(h ^ int(u))
+        h = mix(h ^ int(m))
+    return h
+
+
+class Server:
+    """Continuous-batching request loop (sa_serve_* of include/sa_hip.h)."""
+
+    def __init__(self, hbm_budget=0, max_active=0, chunk=0, digests=True):
+        cfg = ServeConfig(hbm_budget, max_active, chunk, 1 if digests else 0)
+        s = _vp()
+        check(lib().sa_serve_create(C.byref(cfg), C.byref(s)), "sa_serve_create")
+        self.s = s
+
+    def clock(self) -> float:
+        return lib().sa_serve_clock(self.s)
+
+    def submit(self, r: Request) -> int:
+        """Submit; an invalid request is recorded as refused and raises SAError."""
+        i = C.c_int64(-1)
+        check(lib().sa_serve_submit(self.s, C.byref(r), C.byref(i)), "sa_serve_submit")
+        return i.value
+
+    def run(self, n_finished: int = 0):
+        check(lib().sa_serve_run(self.s, n_finished), "sa_serve_run")
+
+    def results(self):
+        n = lib().sa_serve_results(self.s, 0, 1 << 40, None)
+        out = (RequestResult * max(n, 1))()
+        lib().sa_serve_results(self.s, 0, n, out)
+        return list(out)[:n]
+
+    def stats(self) -> dict:
+        st = ServeStats()
+        check(lib().sa_serve_get_stats(self.s, C.byref(st)), "sa_serve_get_stats")
+        return {k: getattr(st, k) for k, _ in ServeStats._fields_}
+
+    def close(self):
+        if getattr(self, "s", None):
+            lib().sa_serve_destroy(self.s)
+            self.s = None
 
     def __del__(self):
         self.close()

@@ -1,19 +1,30 @@
-// sa_batch.hip -- many independent instances of uneven size in one stream of
-// launches (the C ABI sa_batch_* of include/sa_hip.h).
+// sa_batch.hip -- many independent instances of uneven size swept together (the
+// C ABI sa_batch_* of include/sa_hip.h).
 //
-// Each instance is a ladder (couplings, spins, nsys systems) with its own
-// temperature schedule and its own counter-RNG seed.  A half-sweep launch covers
-// every unfinished instance: the grid is the concatenation of the instances' work
-// items (3D: (column, system), 2D: (site, system)); a block belongs to one
-// instance (block map), reads the instance's descriptor and runs the same
-// per-column / per-site code as the single-instance kernels, so an instance's
-// spins are bit for bit those of running it alone.  The instance's sweep index
-// lives in its device descriptor and is advanced by a one-thread-per-instance
-// kernel after the second half, so a chunk of sweeps is a fixed sequence of
-// launches, captured once as a hipGraph and replayed until an instance finishes
-// and the block map changes.  Block size and chunk length are tuned online: the
-// first chunks try the candidates (results do not depend on them) and the
-// fastest is kept.
+// Each instance is a ladder (couplings, spins, nsys systems) with its own counter
+// RNG seed and either an anneal (temperatures per sweep) or parallel tempering
+// (fixed ladder of temperatures, an exchange attempt every `every` sweeps).
+//
+// Sweeps.  A half-sweep launch covers every unfinished instance.  The work items
+// (3D: (column, system), 2D: (site, system)) of all instances are cut into blocks
+// of one instance each; the block map and the per-instance descriptors live in
+// device memory and a fixed grid walks the block map (grid-stride), so the launch
+// sequence of a chunk of sweeps does not depend on which instances are present: it
+// is captured once per chunk length as a hipGraph and replayed while instances
+// come and go.  Every block runs the per-column / per-site code of the
+// single-instance kernels, so an instance's spins are bit for bit those of its
+// ladder swept alone.  A one-block kernel advances the instances' sweep indices
+// after the second half-sweep.
+//
+// Exchanges.  Chunks end on every parallel-tempering instance's exchange points.
+// There the host draws the pair and the uniform of each due instance (its own
+// mt19937(123), as parallel_tempering::step), one launch measures the coldest
+// system and the pair of all due instances, the host decides the lane masks with
+// the reference's rule, and one launch applies all masked swaps.
+//
+// Tuning.  Threads per block and sweeps per graph are picked online: the first
+// chunks try the candidates (results do not depend on them), the fastest per work
+// item is kept.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,6 +33,7 @@
 #include <memory>
 #include <vector>
 
+#include "sa/exchange.hpp"
 #include "sa_internal.hpp"
 
 namespace {
@@ -41,30 +53,193 @@ __device__ __forceinline__ void batch_item(const inst_desc &in, int color, int k
   else sa::half2d_site<true, false, true, 1>(a, color, k, sys);
 }
 
+// ctrl[0] = blocks in the block map, ctrl[1] = descriptors
 __global__ void __launch_bounds__(256) k_batch_half(const inst_desc *__restrict__ d, const int2 *__restrict__ bmap,
-                                                    int color) {
-  const int2 bm = bmap[blockIdx.x];
-  const inst_desc &in = d[bm.x];
-  if (in.t >= in.nsweeps) return;
-  const int L = in.a.L, work = L * (L >> 1);
-  const int item = bm.y + (int)threadIdx.x;
-  if (item >= work * in.a.nsys) return;
-  const int sys = item / work, k = item - sys * work;
-  if (in.fast) batch_item<true>(in, color, k, sys);
-  else batch_item<false>(in, color, k, sys);
+                                                    const int *__restrict__ ctrl, int color) {
+  const int nb = ctrl[0];
+  for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+    const int2 bm = bmap[blk];
+    const inst_desc &in = d[bm.x];
+    if (in.t >= in.nsweeps) continue;
+    const int L = in.a.L, work = L * (L >> 1);
+    const int item = bm.y + (int)threadIdx.x;
+    if (item >= work * in.a.nsys) continue;
+    const int sys = item / work, k = item - sys * work;
+    if (in.fast) batch_item<true>(in, color, k, sys);
+    else batch_item<false>(in, color, k, sys);
+  }
 }
 
-__global__ void k_batch_advance(inst_desc *d, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && d[i].t < d[i].nsweeps) d[i].t += 1;
+__global__ void k_batch_advance(inst_desc *d, const int *__restrict__ ctrl) {
+  const int n = ctrl[1];
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    if (d[i].t < d[i].nsweeps) d[i].t += 1;
 }
+
+// ---- measurement of listed systems of many instances
+// An item is up to kMeasSites consecutive sites of one system (one block of 256
+// threads: 4 waves x 16 groups of 64 sites); counts go to cnt[slot*128 + lane]
+// (satisfied -x,-y[,-z] bonds) and cnt[slot*128 + 64 + lane] (set spins), as in
+// the single-ladder measurement.
+constexpr int kMeasSites = 4096;
+struct meas_item {
+  const uint64_t *s, *jx, *jy, *jz;
+  int64_t N, begin;
+  int kind, L, slot, pad;
+};
+
+template <int KIND>
+__device__ __forceinline__ void measure_item(const meas_item &it, int lane, int wv, int &sat, int &up) {
+  uint64_t e[6] = {0, 0, 0, 0, 0, 0}, mc[5] = {0, 0, 0, 0, 0};
+  const int64_t L = it.L, L2 = L * L, N = it.N;
+  // wave wv takes groups wv, wv+4, ... (16 groups): E <= 48 < 64, M <= 16 < 32
+  for (int g = wv; g < kMeasSites / 64; g += 4) {
+    const int64_t i = it.begin + (int64_t)g * 64 + lane;
+    uint64_t w = 0, bw = 0, bn = 0, bu = 0;
+    if (i < N) {
+      const int64_t x = i % L, y = (i / L) % L, z = i / L2;
+      const int64_t iw = i - x + (x == 0 ? L - 1 : x - 1);
+      const int64_t inn = (y == 0) ? i + L2 - L : i - L;
+      w = it.s[i];
+      bw = (KIND == sa::KIND_2P && x == 0) ? 0 : (it.jx[i] ^ w ^ it.s[iw]);
+      bn = it.jy[i] ^ w ^ it.s[inn];
+      if (KIND == sa::KIND_3D) {
+        const int64_t iu = (z == 0) ? i + N - L2 : i - L2;
+        bu = it.jz[i] ^ w ^ it.s[iu];
+      }
+    }
+    const uint64_t sb = bw ^ bn ^ bu, cb = sa::maj3(bw, bn, bu);
+    uint64_t t = e[0] & sb;
+    e[0] ^= sb;
+    uint64_t u = sa::maj3(e[1], cb, t);
+    e[1] ^= cb ^ t;
+#pragma unroll
+    for (int b = 2; b < 6; ++b) {
+      const uint64_t v = e[b] & u;
+      e[b] ^= u;
+      u = v;
+    }
+    t = w;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+      const uint64_t v = mc[b] & t;
+      mc[b] ^= t;
+      t = v;
+    }
+  }
+  sat = up = 0;
+#pragma unroll
+  for (int b = 0; b < 6; ++b) sat += __popcll(sa::wave_transpose64(e[b], lane)) << b;
+#pragma unroll
+  for (int b = 0; b < 5; ++b) up += __popcll(sa::wave_transpose64(mc[b], lane)) << b;
+}
+
+__global__ void __launch_bounds__(256) k_batch_measure(const meas_item *__restrict__ items, int *cnt) {
+  __shared__ int red[2][64];
+  const meas_item it = items[blockIdx.x];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (threadIdx.x < 128) (&red[0][0])[threadIdx.x] = 0;
+  __syncthreads();
+  int sat = 0, up = 0;
+  if (it.kind == SA_KIND_3D) measure_item<sa::KIND_3D>(it, lane, wv, sat, up);
+  else if (it.kind == SA_KIND_2D) measure_item<sa::KIND_2D>(it, lane, wv, sat, up);
+  else measure_item<sa::KIND_2P>(it, lane, wv, sat, up);
+  atomicAdd(&red[0][lane], sat);
+  atomicAdd(&red[1][lane], up);
+  __syncthreads();
+  if (threadIdx.x < 128) atomicAdd(&cnt[it.slot * 128 + threadIdx.x], (&red[0][0])[threadIdx.x]);
+}
+
+// ---- masked swaps of many pairs: a ^= (a^b)&m, b ^= (a^b)&m (src/device.cuh:18-23)
+constexpr int kSwapWords = 4096;
+struct swap_item {
+  uint64_t *a, *b;
+  uint64_t mask;
+  int64_t begin, end;
+};
+
+__global__ void __launch_bounds__(256) k_batch_swap(const swap_item *__restrict__ items) {
+  const swap_item it = items[blockIdx.x];
+  for (int64_t i = it.begin + threadIdx.x; i < it.end; i += blockDim.x) {
+    const uint64_t x = it.a[i], y = it.b[i];
+    const uint64_t d = (x ^ y) & it.mask;
+    it.a[i] = x ^ d;
+    it.b[i] = y ^ d;
+  }
+}
+
+// pinned host staging that grows on demand (contents are not kept)
+struct pinned {
+  void *p = nullptr;
+  size_t cap = 0;
+  ~pinned() {
+    if (p) (void)hipHostFree(p);
+  }
+  bool ensure(size_t bytes) {
+    if (bytes <= cap) return true;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(bytes * 2, 4096);
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      return false;
+    }
+    cap = want;
+    return true;
+  }
+};
+
+// device array that grows on demand (contents are not kept)
+template <class T>
+struct dev_array {
+  T *p = nullptr;
+  size_t cap = 0;
+  ~dev_array() {
+    if (p) (void)hipFree(p);
+  }
+  // `moved`: the array has a new address (graphs that name it are stale)
+  int ensure(size_t n, bool &moved) {
+    moved = false;
+    if (n <= cap) return SA_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(n * 2, 64);
+    if (hipMalloc(&p, want * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      return sa_fail(SA_ERR_NOMEM, "batch device array of %zu bytes", want * sizeof(T));
+    }
+    cap = want;
+    moved = true;
+    return SA_OK;
+  }
+};
+
+struct pt_state {
+  int every = 0;  // exchange every `every` sweeps; 0 = anneal instance
+  std::vector<double> T;
+  double h = 0;
+  sa::mt19937 gen{};
+  int uni_b = 0;
+  int pend_i = 0;
+  double pend_u = 0;
+  std::vector<int32_t> pairs;
+  std::vector<double> us;
+  std::vector<uint64_t> masks;
+  std::vector<float> E_min;
+  std::vector<double> acc_sum;
+  std::vector<int64_t> acc_n;
+};
 
 struct batch_inst {
   sa_ladder *lad = nullptr;
   sa::sys_table *tabs = nullptr;
   int nsweeps = 0, per_sweep = 0, t = 0;
   bool fast = true, live = false;
-  int64_t work = 0;  // work items per half-sweep (all systems)
+  int64_t work = 0;   // work items per half-sweep (all systems)
+  int64_t bytes = 0;  // device memory held
+  pt_state pt;
 };
 
 const int kBlocks[3] = {64, 128, 256};
@@ -74,29 +249,36 @@ const int kChunks[2] = {8, 32};
 
 struct sa_batch {
   hipStream_t stream = nullptr;
+  int cus = 256;
   std::vector<batch_inst> inst;  // indexed by id
   // device state of the current schedule
-  inst_desc *d_desc = nullptr;
-  int desc_cap = 0, ndesc = 0;
+  dev_array<inst_desc> d_desc;
+  dev_array<int2> d_bmap;
+  dev_array<int> d_ctrl;
+  int ndesc = 0, nblocks = 0, bmap_bs = 0;
   std::vector<int> desc_id;  // instance id of each descriptor
-  int2 *d_bmap = nullptr;
-  int bmap_cap = 0, nblocks = 0, bmap_bs = 0;
   bool dirty = true;
-  hipGraphExec_t exec = nullptr;
-  int exec_k = 0;
+  std::map<int, hipGraphExec_t> graphs;  // chunk length -> graph over the current arrays and block size
+  int graph_bs = 0;
+  // exchange staging
+  pinned h_stage;
+  dev_array<meas_item> d_meas;
+  dev_array<swap_item> d_swap;
+  dev_array<int> d_cnt;
+  pinned h_cnt;
   // tuning: 0 = automatic
   int block = 0, chunk = 0;
   int tuned_block = 0, tuned_chunk = 0;
-  std::vector<std::pair<int, float>> trials;  // (candidate, ms per sweep)
+  std::vector<std::pair<int, float>> trials;  // (candidate, ms per work item)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int64_t bytes_live = 0;
 };
 
 namespace {
 
-void drop_graph(sa_batch *b) {
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  b->exec = nullptr;
-  b->exec_k = 0;
+void drop_graphs(sa_batch *b) {
+  for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);
+  b->graphs.clear();
 }
 
 int check_id(const sa_batch *b, int id) {
@@ -105,7 +287,13 @@ int check_id(const sa_batch *b, int id) {
   return SA_OK;
 }
 
-// descriptors and block map of the unfinished instances, largest first
+int grid_for(const sa_batch *b, int bs) {
+  // two blocks per wave slot at 4 waves/SIMD (the sweep kernel's register
+  // budget): every CU filled, few enough blocks to loop
+  return std::max(1, b->cus * 16 * 2 / (bs / 64));
+}
+
+// descriptors and block map of the unfinished instances, largest remaining work first
 int rebuild(sa_batch *b, int bs) {
   std::vector<int> ids;
   for (int i = 0; i < (int)b->inst.size(); ++i)
@@ -114,14 +302,29 @@ int rebuild(sa_batch *b, int bs) {
     const auto &a = b->inst[x], &c = b->inst[y];
     return a.work * (int64_t)(a.nsweeps - a.t) > c.work * (int64_t)(c.nsweeps - c.t);
   });
-  // earlier launches still read the old descriptors and block map
+  int64_t nblk = 0;
+  for (int id : ids) nblk += (b->inst[id].work + bs - 1) / bs;
+  if (nblk > INT32_MAX / 2) return sa_fail(SA_ERR_ARG, "batch too large");
+  // earlier launches may still read the old arrays and the staging buffer
   SA_HIP_TRY(hipStreamSynchronize(b->stream));
-  std::vector<inst_desc> desc(ids.size());
-  std::vector<int2> bmap;
+  bool m1 = false, m2 = false, m3 = false;
+  int rc = b->d_desc.ensure(std::max<size_t>(ids.size(), 1), m1);
+  if (!rc) rc = b->d_bmap.ensure(std::max<int64_t>(nblk, 1), m2);
+  if (!rc) rc = b->d_ctrl.ensure(2, m3);
+  if (rc) return rc;
+  if (m1 || m2 || m3 || b->graph_bs != bs) drop_graphs(b);
+  b->graph_bs = bs;
+  const size_t bytes = sizeof(int) * 2 + sizeof(inst_desc) * ids.size() + sizeof(int2) * nblk + 64;
+  if (!b->h_stage.ensure(bytes)) return sa_fail(SA_ERR_NOMEM, "pinned staging");
+  int *ctrl = (int *)b->h_stage.p;
+  inst_desc *desc = (inst_desc *)((char *)b->h_stage.p + 16);
+  int2 *bmap = (int2 *)(desc + ids.size());
+  int64_t q2 = 0;
   for (size_t q = 0; q < ids.size(); ++q) {
     const batch_inst &in = b->inst[ids[q]];
     const sa_ladder *l = in.lad;
-    inst_desc &d = desc[q];
+    inst_desc d;
+    std::memset(&d, 0, sizeof d);
     d.a.s = l->s;
     d.a.jx = l->jx;
     d.a.jy = l->jy;
@@ -138,72 +341,184 @@ int rebuild(sa_batch *b, int bs) {
     d.t = in.t;
     d.per_sweep = in.per_sweep;
     d.fast = in.fast ? 1 : 0;
-    d.pad = 0;
-    for (int64_t off = 0; off < in.work; off += bs) bmap.push_back(make_int2((int)q, (int)off));
+    desc[q] = d;
+    for (int64_t off = 0; off < in.work; off += bs) bmap[q2++] = make_int2((int)q, (int)off);
   }
-  if ((int)desc.size() > b->desc_cap) {
-    if (b->d_desc) (void)hipFree(b->d_desc);
-    b->d_desc = nullptr;
-    const int cap = std::max<int>(64, (int)desc.size() * 2);
-    if (hipMalloc(&b->d_desc, sizeof(inst_desc) * cap) != hipSuccess) return sa_fail(SA_ERR_NOMEM, "batch descriptors");
-    b->desc_cap = cap;
+  ctrl[0] = (int)nblk;
+  ctrl[1] = (int)ids.size();
+  SA_HIP_TRY(hipMemcpyAsync(b->d_ctrl.p, ctrl, sizeof(int) * 2, hipMemcpyHostToDevice, b->stream));
+  if (!ids.empty()) {
+    SA_HIP_TRY(hipMemcpyAsync(b->d_desc.p, desc, sizeof(inst_desc) * ids.size(), hipMemcpyHostToDevice, b->stream));
+    SA_HIP_TRY(hipMemcpyAsync(b->d_bmap.p, bmap, sizeof(int2) * nblk, hipMemcpyHostToDevice, b->stream));
   }
-  if ((int)bmap.size() > b->bmap_cap) {
-    if (b->d_bmap) (void)hipFree(b->d_bmap);
-    b->d_bmap = nullptr;
-    const int cap = std::max<int>(1024, (int)bmap.size() * 2);
-    if (hipMalloc(&b->d_bmap, sizeof(int2) * cap) != hipSuccess) return sa_fail(SA_ERR_NOMEM, "batch block map");
-    b->bmap_cap = cap;
-  }
-  if (!desc.empty()) {
-    SA_HIP_TRY(hipMemcpyAsync(b->d_desc, desc.data(), sizeof(inst_desc) * desc.size(), hipMemcpyHostToDevice,
-                              b->stream));
-    SA_HIP_TRY(hipMemcpyAsync(b->d_bmap, bmap.data(), sizeof(int2) * bmap.size(), hipMemcpyHostToDevice, b->stream));
-    SA_HIP_TRY(hipStreamSynchronize(b->stream));  // the host vectors end here
-  }
-  b->ndesc = (int)desc.size();
+  b->ndesc = (int)ids.size();
   b->desc_id = ids;
-  b->nblocks = (int)bmap.size();
+  b->nblocks = (int)nblk;
   b->bmap_bs = bs;
   b->dirty = false;
-  drop_graph(b);
   return SA_OK;
 }
 
 void enqueue_sweeps(sa_batch *b, int k) {
+  const int grid = grid_for(b, b->bmap_bs);
   for (int s = 0; s < k; ++s) {
     for (int color = 0; color < 2; ++color)
-      hipLaunchKernelGGL(k_batch_half, dim3(b->nblocks), dim3(b->bmap_bs), 0, b->stream, b->d_desc, b->d_bmap, color);
-    hipLaunchKernelGGL(k_batch_advance, dim3((b->ndesc + 255) / 256), dim3(256), 0, b->stream, b->d_desc, b->ndesc);
+      hipLaunchKernelGGL(k_batch_half, dim3(grid), dim3(b->bmap_bs), 0, b->stream, b->d_desc.p, b->d_bmap.p,
+                         b->d_ctrl.p, color);
+    hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(256), 0, b->stream, b->d_desc.p, b->d_ctrl.p);
   }
 }
 
-// the graph of k sweeps of the current schedule (captured on first use)
-int prepare_graph(sa_batch *b, int k) {
-  if (b->exec && b->exec_k == k) return SA_OK;
-  drop_graph(b);
+// the graph of k sweeps over the current device arrays (captured on first use)
+int graph_for(sa_batch *b, int k, hipGraphExec_t &out) {
+  auto it = b->graphs.find(k);
+  if (it != b->graphs.end()) {
+    out = it->second;
+    return SA_OK;
+  }
   hipGraph_t g = nullptr;
   SA_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   enqueue_sweeps(b, k);
   SA_HIP_TRY(hipStreamEndCapture(b->stream, &g));
-  const hipError_t e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  hipGraphExec_t e = nullptr;
+  const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
-  if (e != hipSuccess) {
-    b->exec = nullptr;
-    return sa_fail(SA_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-  }
-  b->exec_k = k;
+  if (rc != hipSuccess) return sa_fail(SA_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(rc));
+  b->graphs[k] = e;
+  out = e;
   return SA_OK;
 }
 
-// host mirror of the device sweep indices after k more sweeps
-void advance_host(sa_batch *b, int k) {
+int64_t bonds(const sa_ladder *l) {
+  return (l->kind == SA_KIND_3D ? 3 : 2) * l->N - (l->kind == SA_KIND_2P ? l->L : 0);
+}
+
+// measure systems sys[j] of instances ids[j]: E/M[j*64 + lane]
+int measure_many(sa_batch *b, const std::vector<int> &ids, const std::vector<int> &sys, std::vector<int32_t> &E,
+                 std::vector<int32_t> &M) {
+  const int n = (int)ids.size();
+  E.assign((size_t)n * 64, 0);
+  M.assign((size_t)n * 64, 0);
+  if (n == 0) return SA_OK;
+  std::vector<meas_item> items;
+  for (int j = 0; j < n; ++j) {
+    const sa_ladder *l = b->inst[ids[j]].lad;
+    for (int64_t beg = 0; beg < l->N; beg += kMeasSites) {
+      meas_item it;
+      std::memset(&it, 0, sizeof it);
+      it.s = l->s + (int64_t)sys[j] * l->N;
+      it.jx = l->jx;
+      it.jy = l->jy;
+      it.jz = l->jz;
+      it.N = l->N;
+      it.begin = beg;
+      it.kind = l->kind;
+      it.L = l->L;
+      it.slot = j;
+      items.push_back(it);
+    }
+  }
+  SA_HIP_TRY(hipStreamSynchronize(b->stream));  // the staging buffer may still be read
+  bool moved = false;
+  int rc = b->d_meas.ensure(items.size(), moved);
+  if (!rc) rc = b->d_cnt.ensure((size_t)n * 128, moved);
+  if (rc) return rc;
+  if (!b->h_stage.ensure(sizeof(meas_item) * items.size())) return sa_fail(SA_ERR_NOMEM, "pinned staging");
+  if (!b->h_cnt.ensure(sizeof(int) * n * 128)) return sa_fail(SA_ERR_NOMEM, "pinned staging");
+  std::memcpy(b->h_stage.p, items.data(), sizeof(meas_item) * items.size());
+  SA_HIP_TRY(hipMemcpyAsync(b->d_meas.p, b->h_stage.p, sizeof(meas_item) * items.size(), hipMemcpyHostToDevice,
+                            b->stream));
+  hipLaunchKernelGGL(sa::k_fill_zero, dim3((n * 128 + 255) / 256), dim3(256), 0, b->stream, b->d_cnt.p, n * 128);
+  hipLaunchKernelGGL(k_batch_measure, dim3((unsigned)items.size()), dim3(256), 0, b->stream, b->d_meas.p,
+                     b->d_cnt.p);
+  SA_HIP_TRY(hipGetLastError());
+  SA_HIP_TRY(hipMemcpyAsync(b->h_cnt.p, b->d_cnt.p, sizeof(int) * n * 128, hipMemcpyDeviceToHost, b->stream));
+  SA_HIP_TRY(hipStreamSynchronize(b->stream));
+  const int *cnt = (const int *)b->h_cnt.p;
+  for (int j = 0; j < n; ++j) {
+    const sa_ladder *l = b->inst[ids[j]].lad;
+    const int64_t nb = bonds(l);
+    for (int k = 0; k < 64; ++k) {
+      E[j * 64 + k] = (int32_t)(2 * (int64_t)cnt[j * 128 + k] - nb);
+      M[j * 64 + k] = (int32_t)(2 * (int64_t)cnt[j * 128 + 64 + k] - l->N);
+    }
+  }
+  return SA_OK;
+}
+
+// exchange step of every listed parallel-tempering instance (parallel_tempering::
+// step after its sweeps, src/parallel_tempering.hpp:81-115)
+int pt_exchange(sa_batch *b, const std::vector<int> &due) {
+  if (due.empty()) return SA_OK;
+  std::vector<int> ids, sys;
+  for (int id : due) {
+    batch_inst &in = b->inst[id];
+    in.pt.pend_i = sa::uniform_int(in.pt.gen, 0, in.pt.uni_b);
+    in.pt.pend_u = sa::canonical32(in.pt.gen);
+    for (int s : {0, in.pt.pend_i, in.pt.pend_i + 1}) {
+      ids.push_back(id);
+      sys.push_back(s);
+    }
+  }
+  std::vector<int32_t> E, M;
+  int rc = measure_many(b, ids, sys, E, M);
+  if (rc) return rc;
+  std::vector<swap_item> items;
+  for (size_t q = 0; q < due.size(); ++q) {
+    batch_inst &in = b->inst[due[q]];
+    pt_state &p = in.pt;
+    float e[3][64];
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 64; ++k)
+        e[r][k] = (float)((double)E[(q * 3 + r) * 64 + k] + p.h * (double)M[(q * 3 + r) * 64 + k]);
+    if (p.E_min.empty()) p.E_min.assign(e[0], e[0] + 64);
+    for (int k = 0; k < 64; ++k) p.E_min[k] = std::max(e[0][k], p.E_min[k]);
+    const int i = p.pend_i;
+    const double dB = 1. / p.T[i] - 1. / p.T[i + 1];
+    int acc = 0;
+    const uint64_t mask = sa::exchange_word(e[1], e[2], dB, p.pend_u, 64, acc);
+    p.acc_sum[i] += acc / 64.0;
+    p.acc_n[i] += 1;
+    p.pairs.push_back(i);
+    p.us.push_back(p.pend_u);
+    p.masks.push_back(mask);
+    if (!mask) continue;
+    const sa_ladder *l = in.lad;
+    for (int64_t beg = 0; beg < l->N; beg += kSwapWords) {
+      swap_item it;
+      it.a = l->s + (int64_t)i * l->N;
+      it.b = l->s + (int64_t)(i + 1) * l->N;
+      it.mask = mask;
+      it.begin = beg;
+      it.end = std::min<int64_t>(l->N, beg + kSwapWords);
+      items.push_back(it);
+    }
+  }
+  if (items.empty()) return SA_OK;
+  bool moved = false;
+  rc = b->d_swap.ensure(items.size(), moved);
+  if (rc) return rc;
+  if (!b->h_stage.ensure(sizeof(swap_item) * items.size())) return sa_fail(SA_ERR_NOMEM, "pinned staging");
+  std::memcpy(b->h_stage.p, items.data(), sizeof(swap_item) * items.size());
+  SA_HIP_TRY(hipMemcpyAsync(b->d_swap.p, b->h_stage.p, sizeof(swap_item) * items.size(), hipMemcpyHostToDevice,
+                            b->stream));
+  hipLaunchKernelGGL(k_batch_swap, dim3((unsigned)items.size()), dim3(256), 0, b->stream, b->d_swap.p);
+  SA_HIP_TRY(hipGetLastError());
+  // every later writer of the staging buffer synchronises the stream first
+  return SA_OK;
+}
+
+// host mirror of the device sweep indices after k more sweeps; collects the
+// parallel-tempering instances whose exchange is due
+void advance_host(sa_batch *b, int k, std::vector<int> &due) {
   for (int id : b->desc_id) {
     batch_inst &in = b->inst[id];
+    if (!in.live) continue;
     const int before = in.t;
     in.t = std::min(in.nsweeps, in.t + k);
     in.lad->t = (uint64_t)in.t;
     if (in.t >= in.nsweeps && before < in.nsweeps) b->dirty = true;
+    if (in.pt.every && in.t > before && in.t % in.pt.every == 0) due.push_back(id);
   }
 }
 
@@ -212,43 +527,8 @@ int sync_batch(sa_batch *b) {
   return SA_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int sa_batch_create(sa_batch **out) {
-  if (!out) return sa_fail(SA_ERR_ARG, "null output handle");
-  sa_batch *b = new sa_batch();
-  if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
-      hipEventCreate(&b->ev1) != hipSuccess) {
-    delete b;
-    return sa_fail(SA_ERR_HIP, "stream/event creation failed");
-  }
-  *out = b;
-  return SA_OK;
-}
-
-int sa_batch_destroy(sa_batch *b) {
-  if (!b) return SA_OK;
-  (void)hipStreamSynchronize(b->stream);
-  drop_graph(b);
-  for (auto &in : b->inst)
-    if (in.live) {
-      if (in.tabs) (void)hipFree(in.tabs);
-      sa_ladder_destroy(in.lad);
-    }
-  if (b->d_desc) (void)hipFree(b->d_desc);
-  if (b->d_bmap) (void)hipFree(b->d_bmap);
-  if (b->ev0) (void)hipEventDestroy(b->ev0);
-  if (b->ev1) (void)hipEventDestroy(b->ev1);
-  (void)hipStreamDestroy(b->stream);
-  delete b;
-  return SA_OK;
-}
-
-int sa_batch_add(sa_batch *b, int kind, int L, int nsys, uint64_t seed, int nsweeps, const double *T, int per_sweep,
-                 double h, int *id) {
-  if (!b || !T || !id || nsweeps < 0) return sa_fail(SA_ERR_ARG, "bad argument");
+int add_instance(sa_batch *b, int kind, int L, int nsys, uint64_t seed, int nsweeps, const double *T, int per_sweep,
+                 double h, pt_state *pt, int *id) {
   sa_ladder *l = nullptr;
   int rc = sa_ladder_create(kind, L, nsys, &l);
   if (rc) return rc;
@@ -275,72 +555,167 @@ int sa_batch_add(sa_batch *b, int kind, int L, int nsys, uint64_t seed, int nswe
   if (!rc && hipMalloc(&in.tabs, sizeof(sa::sys_table) * ntab) != hipSuccess) rc = sa_fail(SA_ERR_NOMEM, "tables");
   if (!rc && hipMemcpy(in.tabs, tabs.data(), sizeof(sa::sys_table) * ntab, hipMemcpyHostToDevice) != hipSuccess)
     rc = sa_fail(SA_ERR_HIP, "table upload");
+  // fixed temperatures also become the ladder's own tables, for plain sweeps later
+  if (!rc && !per_sweep)
+    for (int j = 0; j < nsys && !rc; ++j) rc = sa_ladder_set_temperature(l, j, T[j], h, nullptr);
   if (!rc) rc = sa_ladder_sync(l);
   if (rc) {
     if (in.tabs) (void)hipFree(in.tabs);
     sa_ladder_destroy(l);
     return rc;
   }
-  // fixed temperatures also become the ladder's own tables, for plain sweeps later
-  if (!per_sweep)
-    for (int j = 0; j < nsys; ++j) sa_ladder_set_temperature(l, j, T[j], h, nullptr);
+  in.bytes = sa_detail::batch_instance_bytes(kind, L, nsys, nsweeps, per_sweep);
+  if (pt) in.pt = *pt;
   in.live = true;
-  b->inst.push_back(in);
+  b->bytes_live += in.bytes;
+  b->inst.push_back(std::move(in));
   *id = (int)b->inst.size() - 1;
   b->dirty = true;
   return SA_OK;
 }
 
+}  // namespace
+
+namespace sa_detail {
+int64_t batch_instance_bytes(int kind, int L, int nsys, int nsweeps, int per_sweep) {
+  const int ntab = per_sweep ? std::max(1, nsweeps) * nsys : nsys;
+  return ladder_bytes(kind, L, nsys) + (int64_t)sizeof(sa::sys_table) * ntab;
+}
+
+int batch_launch_chunk(sa_batch *b, int max_sweeps, int *k_out) {
+  *k_out = 0;
+  const int ncand = 3 * 2;
+  // block size and chunk length: fixed by the caller, tuned, or the next trial
+  int trial = -1;
+  if ((!b->block || !b->chunk) && !(b->tuned_block && b->tuned_chunk)) {
+    trial = (int)b->trials.size();
+    if (trial >= ncand) {
+      int best = 0;
+      for (int q = 1; q < ncand; ++q)
+        if (b->trials[q].second < b->trials[best].second) best = q;
+      b->tuned_block = kBlocks[best / 2];
+      b->tuned_chunk = kChunks[best % 2];
+      trial = -1;
+    }
+  }
+  const int bs = b->block ? b->block : (trial >= 0 ? kBlocks[trial / 2] : b->tuned_block);
+  const int kfull = b->chunk ? b->chunk : (trial >= 0 ? kChunks[trial % 2] : b->tuned_chunk);
+  if (b->dirty || b->bmap_bs != bs) {
+    const int rc = rebuild(b, bs);
+    if (rc) return rc;
+  }
+  if (b->ndesc == 0 || max_sweeps <= 0) return SA_OK;
+  // the chunk ends at the next exchange point of any parallel-tempering instance
+  int64_t k = std::min(kfull, max_sweeps);
+  int remaining = 0;
+  for (int id : b->desc_id) {
+    const batch_inst &in = b->inst[id];
+    remaining = std::max(remaining, in.nsweeps - in.t);
+    if (in.pt.every) k = std::min<int64_t>(k, in.pt.every - in.t % in.pt.every);
+  }
+  k = std::min<int64_t>(k, remaining);
+  if (k <= 0) return SA_OK;
+  hipGraphExec_t g = nullptr;
+  int rc = graph_for(b, (int)k, g);
+  if (rc) return rc;
+  if (trial >= 0) SA_HIP_TRY(hipEventRecord(b->ev0, b->stream));
+  SA_HIP_TRY(hipGraphLaunch(g, b->stream));
+  if (trial >= 0) {
+    SA_HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    SA_HIP_TRY(hipEventSynchronize(b->ev1));
+    float ms = 0;
+    SA_HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    // time per work item, so trials on different schedules compare
+    double work = 0;
+    for (int id : b->desc_id)
+      work += (double)b->inst[id].work * std::min<int64_t>(k, b->inst[id].nsweeps - b->inst[id].t);
+    b->trials.push_back({trial, (float)(ms / std::max(work, 1.0))});
+  }
+  *k_out = (int)k;
+  return SA_OK;
+}
+
+int batch_finish_chunk(sa_batch *b, int k) {
+  std::vector<int> due;
+  advance_host(b, k, due);
+  return pt_exchange(b, due);
+}
+
+int batch_wait(sa_batch *b) { return sync_batch(b); }
+
+bool batch_done(const sa_batch *b, int id) {
+  return id >= 0 && id < (int)b->inst.size() && b->inst[id].live && b->inst[id].t >= b->inst[id].nsweeps;
+}
+}  // namespace sa_detail
+
+extern "C" {
+
+int sa_batch_create(sa_batch **out) {
+  if (!out) return sa_fail(SA_ERR_ARG, "null output handle");
+  sa_batch *b = new sa_batch();
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+      prop.multiProcessorCount > 0)
+    b->cus = prop.multiProcessorCount;
+  if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
+      hipEventCreate(&b->ev1) != hipSuccess) {
+    delete b;
+    return sa_fail(SA_ERR_HIP, "stream/event creation failed");
+  }
+  *out = b;
+  return SA_OK;
+}
+
+int sa_batch_destroy(sa_batch *b) {
+  if (!b) return SA_OK;
+  (void)hipStreamSynchronize(b->stream);
+  drop_graphs(b);
+  for (auto &in : b->inst)
+    if (in.live) {
+      if (in.tabs) (void)hipFree(in.tabs);
+      sa_ladder_destroy(in.lad);
+    }
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  hipStream_t s = b->stream;
+  delete b;  // device arrays and staging buffers free themselves
+  (void)hipStreamDestroy(s);
+  return SA_OK;
+}
+
+int sa_batch_add(sa_batch *b, int kind, int L, int nsys, uint64_t seed, int nsweeps, const double *T, int per_sweep,
+                 double h, int *id) {
+  if (!b || !T || !id || nsweeps < 0 || nsys < 1) return sa_fail(SA_ERR_ARG, "bad argument");
+  return add_instance(b, kind, L, nsys, seed, nsweeps, T, per_sweep, h, nullptr, id);
+}
+
+int sa_batch_add_pt(sa_batch *b, int kind, int L, int nT, const double *T, double h, uint64_t seed, int every,
+                    int nsteps, int *id) {
+  if (!b || !T || !id || nT < 2 || every < 1 || nsteps < 0) return sa_fail(SA_ERR_ARG, "bad argument");
+  if ((int64_t)every * nsteps > INT32_MAX / 2) return sa_fail(SA_ERR_ARG, "too many sweeps");
+  pt_state p;
+  p.every = every;
+  p.T.assign(T, T + nT);
+  std::sort(p.T.begin(), p.T.end());  // parallel_tempering sorts its temperatures
+  p.h = h;
+  p.gen.seed(123);
+  p.uni_b = nT - 2;
+  p.acc_sum.assign(nT - 1, 0.0);
+  p.acc_n.assign(nT - 1, 0);
+  return add_instance(b, kind, L, nT, seed, every * nsteps, p.T.data(), 0, h, &p, id);
+}
+
 int sa_batch_run(sa_batch *b, int sweeps) {
   if (!b) return sa_fail(SA_ERR_ARG, "null batch");
-  const int ncand = 3 * 2;
   int64_t left = sweeps > 0 ? sweeps : INT64_MAX;
   while (left > 0) {
-    // block size and chunk length: fixed by the caller, tuned, or the next trial
-    int trial = -1;
-    if ((!b->block || !b->chunk) && !(b->tuned_block && b->tuned_chunk)) {
-      trial = (int)b->trials.size();
-      if (trial >= ncand) {
-        int best = 0;
-        for (int q = 1; q < ncand; ++q)
-          if (b->trials[q].second < b->trials[best].second) best = q;
-        b->tuned_block = kBlocks[best / 2];
-        b->tuned_chunk = kChunks[best % 2];
-        continue;
-      }
-    }
-    const int bs = b->block ? b->block : (trial >= 0 ? kBlocks[trial / 2] : b->tuned_block);
-    const int kfull = b->chunk ? b->chunk : (trial >= 0 ? kChunks[trial % 2] : b->tuned_chunk);
-    if (b->dirty || b->bmap_bs != bs) {
-      const int rc = rebuild(b, bs);
-      if (rc) return rc;
-    }
-    if (b->ndesc == 0) break;
-    int remaining = 0;
-    for (int id : b->desc_id) remaining = std::max(remaining, b->inst[id].nsweeps - b->inst[id].t);
-    const int k = (int)std::min<int64_t>({(int64_t)kfull, left, (int64_t)remaining});
-    if (k == kfull) {
-      const int rc = prepare_graph(b, k);
-      if (rc) return rc;
-    }
-    if (trial >= 0) SA_HIP_TRY(hipEventRecord(b->ev0, b->stream));
-    if (k == kfull) {
-      SA_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
-    } else {  // a short tail: plain launches
-      enqueue_sweeps(b, k);
-      SA_HIP_TRY(hipGetLastError());
-    }
-    if (trial >= 0) {
-      SA_HIP_TRY(hipEventRecord(b->ev1, b->stream));
-      SA_HIP_TRY(hipEventSynchronize(b->ev1));
-      float ms = 0;
-      SA_HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-      // time per work item, so trials on different schedules compare
-      double work = 0;
-      for (int id : b->desc_id) work += (double)b->inst[id].work * std::min(k, b->inst[id].nsweeps - b->inst[id].t);
-      b->trials.push_back({trial, (float)(ms / std::max(work, 1.0))});
-    }
-    advance_host(b, k);
+    int k = 0;
+    int rc = sa_detail::batch_launch_chunk(b, (int)std::min<int64_t>(left, INT32_MAX), &k);
+    if (rc) return rc;
+    if (k == 0) break;
+    rc = sa_detail::batch_finish_chunk(b, k);
+    if (rc) return rc;
     left -= k;
   }
   int rc = sync_batch(b);
@@ -375,6 +750,25 @@ int sa_batch_measure(sa_batch *b, int id, int32_t *E, int32_t *M) {
   return sa_ladder_measure(b->inst[id].lad, E, M);
 }
 
+int sa_batch_pt_result(sa_batch *b, int id, int64_t capacity, int32_t *pairs, double *us, uint64_t *masks,
+                       float *E_min, double *acceptance) {
+  int rc = check_id(b, id);
+  if (rc) return rc;
+  const pt_state &p = b->inst[id].pt;
+  if (!p.every) return sa_fail(SA_ERR_STATE, "instance %d is not a parallel-tempering instance", id);
+  const int64_t n = std::min<int64_t>(capacity, (int64_t)p.pairs.size());
+  for (int64_t k = 0; k < n; ++k) {
+    if (pairs) pairs[k] = p.pairs[k];
+    if (us) us[k] = p.us[k];
+    if (masks) masks[k] = p.masks[k];
+  }
+  if (E_min)
+    for (int k = 0; k < 64; ++k) E_min[k] = p.E_min.empty() ? 0.f : p.E_min[k];
+  if (acceptance)
+    for (size_t i = 0; i < p.acc_sum.size(); ++i) acceptance[i] = p.acc_n[i] ? p.acc_sum[i] / p.acc_n[i] : 0.0;
+  return (int)std::min<int64_t>((int64_t)p.pairs.size(), INT32_MAX);
+}
+
 sa_ladder *sa_batch_ladder(sa_batch *b, int id) {
   if (check_id(b, id)) return nullptr;
   if (sync_batch(b)) return nullptr;
@@ -390,10 +784,13 @@ int sa_batch_remove(sa_batch *b, int id) {
   batch_inst &in = b->inst[id];
   if (in.tabs) (void)hipFree(in.tabs);
   sa_ladder_destroy(in.lad);
+  b->bytes_live -= in.bytes;
   in = batch_inst();
   b->dirty = true;
   return SA_OK;
 }
+
+int64_t sa_batch_bytes(const sa_batch *b) { return b ? b->bytes_live : 0; }
 
 void *sa_batch_stream(sa_batch *b) { return b ? (void *)b->stream : nullptr; }
 
@@ -409,7 +806,6 @@ int sa_batch_set_tuning(sa_batch *b, int key, int value) {
   } else {
     return sa_fail(SA_ERR_ARG, "unknown batch tuning key %d", key);
   }
-  drop_graph(b);
   b->dirty = true;
   return SA_OK;
 }

@@ -20,6 +20,8 @@ int sa_fail(int status, const char *fmt, ...);
     if (e_ != hipSuccess) return sa_fail(SA_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
+struct sa_batch;
+
 struct sa_ladder {
   int kind = 0, L = 0, nsys = 0, dim = 0;
   int64_t N = 0;
@@ -48,4 +50,17 @@ namespace sa_detail {
 // acceptance thresholds of one temperature/field for a ladder of `kind`; fast =
 // the table qualifies for the symmetric low-levels-always kernels
 void make_table(int kind, double T, double h, sa::sys_table &tab, bool &fast, double *boltz_out);
+// device bytes sa_ladder_create allocates for a ladder (the fused 3D sweep's second
+// spin array, allocated on first use, not included)
+int64_t ladder_bytes(int kind, int L, int nsys);
+// device bytes of a batch instance: its ladder plus its temperature tables
+int64_t batch_instance_bytes(int kind, int L, int nsys, int nsweeps, int per_sweep);
+// One chunk of a batch in two halves, so a caller can do host work while it runs:
+// launch (asynchronous; *k = sweeps launched, 0 when nothing is left), then
+// finish (host sweep indices, exchange steps; waits for the chunk when an
+// exchange is due).  batch_wait waits for everything launched.
+int batch_launch_chunk(sa_batch *b, int max_sweeps, int *k);
+int batch_finish_chunk(sa_batch *b, int k);
+int batch_wait(sa_batch *b);
+bool batch_done(const sa_batch *b, int id);
 }  // namespace sa_detail

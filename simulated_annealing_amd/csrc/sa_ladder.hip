@@ -405,6 +405,11 @@ int sa_ladder_get_spins(const sa_ladder *l, int sys, uint64_t *s) {
 
 }  // extern "C"
 
+int64_t sa_detail::ladder_bytes(int kind, int L, int nsys) {
+  const int64_t N = (int64_t)L * L * (kind == SA_KIND_3D ? L : 1);
+  return (int64_t)nsys * N * 8 + 3 * N * 8 + (int64_t)sizeof(sa::sys_table) * nsys + 4 * 128 * nsys + 4 * nsys;
+}
+
 void sa_detail::make_table(int kind, double T, double h, sa::sys_table &t, bool &fast, double *boltz_out) {
   double b[20];
   t = sa::sys_table{};

@@ -175,3 +175,53 @@ def test_uneven_mix_of_64_instances(sa):
     assert b.run() == 0
     check_mix(sa, b, insts)
     b.close()
+
+
+# (kind, L, nT, every, nsteps, h)
+PT_MIX = [
+    (2, 8, 4, 5, 6, 0.0),
+    (1, 16, 6, 10, 4, 0.0),
+    (0, 12, 3, 7, 5, 0.2),
+    (2, 6, 8, 10, 3, -0.3),
+    (1, 32, 5, 3, 9, 0.0),
+]
+
+
+def test_batch_pt_matches_pt_driver(sa):
+    """Parallel-tempering instances mixed with anneals: each has the swap sequence
+    (pairs, uniforms, lane masks) and final spins of the PT driver run alone."""
+    b = sa.Batch()
+    anneals = build_mix(sa, b, MIX[:5], 31)
+    pts = []
+    for q, (kind, L, nT, every, nsteps, h) in enumerate(PT_MIX):
+        T = 0.5 * 4.0 ** (np.arange(nT)[::-1] / (nT - 1))  # descending: the driver sorts
+        seed = 500 + q
+        i = b.add_pt(kind, L, T, seed, every, nsteps, h=h)
+        pts.append((i, kind, L, nT, every, nsteps, h, T, seed))
+    assert b.run() == 0
+    check_mix(sa, b, anneals)
+    for (i, kind, L, nT, every, nsteps, h, T, seed) in pts:
+        say("batch PT vs driver", "kind", kind, "L", L, "nT", nT, "every", every, "steps", nsteps, "h", h)
+        assert b.sweeps_done(i) == every * nsteps
+        ref = sa.PT(kind, L, T, h, seed, every, sa.RNG_COUNTER)
+        ref.step(nsteps)
+        r = ref.result()
+        got = b.pt_result(i)
+        assert (got["pairs"] == r["pairs"]).all()
+        assert (got["us"] == r["us"]).all()
+        assert (got["masks"] == r["masks"]).all()
+        for j in range(nT):
+            assert (b.spins(i, j) == r["spins"][j]).all(), j
+        # energies now (the driver's E keeps the pre-exchange values of the last pair,
+        # as the reference's E_ does): measure the driver's ladder directly
+        E, M = b.measure(i)
+        E_ref = np.zeros(nT * 64, dtype=np.int32)
+        M_ref = np.zeros(nT * 64, dtype=np.int32)
+        sa.check(sa.lib().sa_ladder_measure(ref.ladder_handle(), E_ref, M_ref), "measure")
+        assert (E.reshape(-1) == E_ref).all() and (M.reshape(-1) == M_ref).all()
+        # systems outside the last exchanged pair agree with the driver's E as well
+        last = int(got["pairs"][-1])
+        keep = [j for j in range(nT) if j not in (last, last + 1)]
+        assert ((E[keep] + h * M[keep]).astype(np.float32) == r["E"][keep]).all()
+        ref.close()
+    b.close()
