@@ -138,6 +138,45 @@ def batched_vs_plain(sa, n=512):
     return out
 
 
+def serving(sa, n=1000, rate=40.0, budget=0, max_active=0):
+    """The serving form: a Poisson stream of anneal / parallel-tempering requests.
+
+    Requests (simulated_annealing_amd.workloads.serving_requests): 2D L log-uniform
+    in [16, 256] or 3D L log-uniform in [8, 64] (even L), sweeps log-uniform in
+    [100, 10000], half of them PT with 8..32 temperatures geometric in [0.5, 2.0]
+    exchanging every 10 sweeps, the others a linear anneal 3.0 -> 0.1; per-request
+    seeds.  All requests are submitted up front with their arrival times; the
+    server admits each when it has arrived and memory / a slot is free.  Latency =
+    result ready (spins digested, energies read back) - arrival."""
+    from simulated_annealing_amd import workloads
+
+    W = workloads.serving_requests(n, rate=rate)
+    srv = sa.Server(hbm_budget=budget, max_active=max_active)
+    base = srv.clock() + 0.25
+    for w in W:
+        srv.submit(sa.make_request(w.kind, w.L, w.mode, w.nsweeps, w.seed, nT=w.nT, every=10,
+                                   arrival=base + w.arrival))
+    srv.run()
+    res = srv.results()
+    st = srv.stats()
+    srv.close()
+    ok = [x for x in res if x.status == 0]
+    lat = np.array([x.finished - x.arrival for x in ok])
+    wait = np.array([x.admitted - x.arrival for x in ok])
+    span = max(x.finished for x in ok) - min(x.arrival for x in ok)
+    attempts = sum(x.attempts for x in ok)
+    return dict(workload=f"{n} Poisson requests at {rate:g}/s: 2D L~logU[16,256] / 3D L~logU[8,64], "
+                         "anneal or PT (8..32 T), sweeps~logU[100,10000]",
+                requests=n, completed=len(ok), rate_per_s=rate,
+                p50_latency_s=round(float(np.percentile(lat, 50)), 4),
+                p99_latency_s=round(float(np.percentile(lat, 99)), 4),
+                mean_latency_s=round(float(lat.mean()), 4),
+                p99_queue_wait_s=round(float(np.percentile(wait, 99)), 4),
+                attempts_per_ns=round(attempts / span / 1e9, 1), span_s=round(span, 3),
+                gpu_busy_frac=round(st["busy_s"] / span, 3), peak_active=st["peak_active"],
+                peak_bytes=st["peak_bytes"], hbm_budget=st["hbm_budget"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +184,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-requests", type=int, default=1000, help="requests in the serving measurement")
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-only", action="store_true", help="print only the serving measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the masked-swap kernel on two 128 MiB systems (known traffic: 256 MiB read, "
                          "256 MiB written per launch) to scale the FETCH_SIZE/WRITE_SIZE counters")
@@ -166,6 +209,9 @@ def main():
 
     import simulated_annealing_amd as sa
 
+    if args.serve_only:
+        print(json.dumps({"serving": serving(sa, args.serve_requests, args.serve_rate)}), flush=True)
+        return
     pt = sa.PT(KIND, L, T_LADDER, 0.0, 1234, SWEEPS, sa.RNG_COUNTER)
     pt.step(args.warmup)
     pt.sync()
@@ -210,6 +256,7 @@ def main():
                     kernel=roofline_kernel(fused), launch_us=round(launch_us, 2), alg_bytes_per_launch=alg_bytes)
     pt.close()
     batched = None if args.no_batch else batched_vs_plain(sa)
+    served = None if args.no_serve else serving(sa, args.serve_requests, args.serve_rate)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "replica spin-flip attempts/ns (1 GPU, 3D L=128); batched vs plain speedup",
@@ -231,6 +278,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched_vs_plain": batched,
+        "serving": served,
     }
     print(json.dumps(line), flush=True)
 
