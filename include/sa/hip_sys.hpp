@@ -190,8 +190,10 @@ class hip_sys : public gpu_sys {
   }
   // Record: int L; double T, h; s[N]; Jx[2N]; Jy[2N]; [Jz[2N]]; the engine
   // (312 words + position); uniform_real (0,1) and uniform_int (0,N-1) parameters
-  // -- the field order of cpu_3d::save (src/cpu_3d.cpp:449-461) -- then a trailer
-  // "SAHIPX01", rng mode, counter seed and sweep index.
+  // -- the bytes of cpu_3d::save / cpu_2dp::save (src/cpu_3d.cpp:449-461,
+  // src/cpu_2dp.cpp:444-455).  With the counter RNG a trailer follows ("SAHIPX01",
+  // rng mode, counter seed, sweep index): that state has no place in the reference's
+  // record.  With the replay RNG the record is the reference's byte for byte.
   std::ostream &save(std::ostream &os) override {
     int L = lad_->L();
     os.write(reinterpret_cast<const char *>(&L), sizeof L);
@@ -207,6 +209,7 @@ class hip_sys : public gpu_sys {
     const int ui[2] = {0, (int)(get_N() - 1)};
     os.write(reinterpret_cast<const char *>(ud), sizeof ud);
     os.write(reinterpret_cast<const char *>(ui), sizeof ui);
+    if (rng_mode_ == SA_RNG_REPLAY) return os;
     uint64_t trailer[4] = {0, (uint64_t)rng_mode_, 0, 0};
     std::memcpy(&trailer[0], "SAHIPX01", 8);
     sa_ladder_get_sweep_index(l(), &trailer[3]);
@@ -241,12 +244,24 @@ class hip_sys : public gpu_sys {
     int ui[2];
     is.read(reinterpret_cast<char *>(ud), sizeof ud);
     is.read(reinterpret_cast<char *>(ui), sizeof ui);
-    uint64_t trailer[4];
-    is.read(reinterpret_cast<char *>(trailer), sizeof trailer);
-    if (!is || std::memcmp(&trailer[0], "SAHIPX01", 8) != 0) {
-      std::cerr << "Error:   load: truncated or foreign system record\n";
-      is.setstate(std::ios::failbit);
+    if (!is) {
+      std::cerr << "Error:   load: truncated system record\n";
       return is;
+    }
+    // optional trailer (counter RNG state); a reference record has none
+    uint64_t trailer[4] = {0, (uint64_t)SA_RNG_REPLAY, 0, 0};
+    const std::streampos mark = is.tellg();
+    char magic[8];
+    is.read(magic, 8);
+    if (is && std::memcmp(magic, "SAHIPX01", 8) == 0) {
+      is.read(reinterpret_cast<char *>(&trailer[1]), 3 * sizeof(uint64_t));
+      if (!is) {
+        std::cerr << "Error:   load: truncated system record\n";
+        return is;
+      }
+    } else {
+      is.clear();
+      is.seekg(mark);
     }
     T_ = T;
     h_ = h;
@@ -255,8 +270,10 @@ class hip_sys : public gpu_sys {
     check("load", sa_ladder_set_couplings(l(), j[0].data(), j[1].data(), dim() == 3 ? j[2].data() : nullptr));
     check("load", sa_ladder_set_engine(l(), eng));
     set_rng((int)trailer[1]);
-    seed_ = trailer[2];
-    check("load", sa_ladder_set_sweep_index(l(), trailer[3]));
+    if (trailer[1] != SA_RNG_REPLAY) {
+      seed_ = trailer[2];
+      check("load", sa_ladder_set_sweep_index(l(), trailer[3]));
+    }
     return is;
   }
   double get_T() override { return T_; }

@@ -236,6 +236,7 @@ class parallel_tempering {
       r(E_[i].data(), m * sizeof(float));
       sys_[i]->load(is);
     }
+    stale_.assign(sys_.size(), 0);  // the loaded energies are current
     return is;
   }
 

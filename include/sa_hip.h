@@ -180,6 +180,13 @@ int64_t sa_pt_steps_done(const sa_pt *pt);
 int sa_pt_result(sa_pt *pt, int64_t capacity, int32_t *pairs, double *us, uint64_t *masks, float *E_last,
                  uint64_t *spins);
 sa_ladder *sa_pt_ladder(sa_pt *pt);
+/* Checkpoint (.tempsys) of the run: parallel_tempering::save / load
+ * (src/parallel_tempering.hpp:280-377; main_cpu's -s / --reload files).  With the
+ * replay RNG the file is the reference's byte for byte and a reference checkpoint
+ * loads and continues identically; with the counter RNG each system record carries
+ * a trailer with the counter state.  Loading clears the swap log. */
+int sa_pt_save(sa_pt *pt, const char *path);
+int sa_pt_load(sa_pt *pt, const char *path);
 int sa_pt_destroy(sa_pt *pt);
 
 /* ---- batched instances (no counterpart in the reference) --------------------

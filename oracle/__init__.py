@@ -270,6 +270,9 @@ def ref():
         R.ref_pt_draws.argtypes = [C.c_int, _i32p, _f64p, C.c_long]
         R.ref_pt_run.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, C.c_double, C.c_long, C.c_int, C.c_int,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        R.ref_pt_checkpoint.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, C.c_double, C.c_long, C.c_int, C.c_int,
+                                        C.c_int, C.c_char_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]
         R.ref_anneal_geometric.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                            C.c_double, C.c_long, C.c_char_p, C.c_long]
         R.ref_time_sweeps.argtypes = [C.c_int, C.c_int, C.c_double, C.c_long, C.c_int]
@@ -342,6 +345,22 @@ def ref_pt_run(kind, L, T, h, seed, sweeps, nsteps):
     ref().ref_pt_run(kind, L, nT, T, h, seed, sweeps, nsteps, _ptr(pairs), _ptr(us), _ptr(masks), _ptr(acc),
                      _ptr(E), _ptr(spins))
     return dict(pairs=pairs, us=us, masks=masks, accept=acc, E=E.reshape(nT, 64), spins=spins.reshape(nT, N))
+
+
+def ref_pt_checkpoint(kind, L, T, h, seed, sweeps, nsteps1, nsteps2, fname, path):
+    """The reference's PT run: nsteps1 steps, parallel_tempering::save to `path`,
+    nsteps2 more steps (their pairs, uniforms, masks) and the final spins."""
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    nT = T.size
+    N = nsites(kind, L)
+    pairs = np.zeros(max(nsteps2, 1), dtype=np.int32)
+    us = np.zeros(max(nsteps2, 1), dtype=np.float64)
+    masks = np.zeros(max(nsteps2, 1), dtype=np.uint64)
+    spins = np.zeros(nT * N, dtype=np.uint64)
+    rc = ref().ref_pt_checkpoint(kind, L, nT, T, h, seed, sweeps, nsteps1, nsteps2, fname.encode(),
+                                 str(path).encode(), _ptr(pairs), _ptr(us), _ptr(masks), _ptr(spins))
+    assert rc == 0
+    return dict(pairs=pairs[:nsteps2], us=us[:nsteps2], masks=masks[:nsteps2], spins=spins.reshape(nT, N))
 
 
 def ref_anneal_geometric(kind, L, T0, alfa, sweeps, nsteps, stride, h, seed) -> str:

@@ -177,6 +177,54 @@ int ref_pt_run(int kind, int L, int nT, const double *Tin, double h, long seed, 
   return 0;
 }
 
+// Checkpoint of a parallel-tempering run as main_cpu writes it (.tempsys,
+// parallel_tempering::save, src/parallel_tempering.hpp:280-315): nsteps1 x step(0),
+// save to `path`, then nsteps2 more steps recording pair, shared uniform and mask
+// per step and the final spins of every system.
+int ref_pt_checkpoint(int kind, int L, int nT, const double *Tin, double h, long seed, int sweeps, int nsteps1,
+                      int nsteps2, const char *fname, const char *path, int32_t *pairs, double *us,
+                      uint64_t *masks, uint64_t *spins) {
+  Logger::verbosity = 0;
+  std::vector<cpu_sys *> sys;
+  for (int i = 0; i < nT; ++i) {
+    cpu_sys *q = make_sys(kind, L, Tin[0], h);
+    q->set_seed(seed);
+    q->init_J();
+    q->set_seed(seed);
+    q->init_rand();
+    sys.push_back(q);
+  }
+  std::vector<double> T(Tin, Tin + nT);
+  std::string fn = fname;
+  parallel_tempering<cpu_sys> pt(sys, T, sweeps, fn);
+  for (int k = 0; k < nsteps1; ++k) pt.step(0);
+  {
+    std::ofstream f(path, std::ios::binary);
+    pt.save(f);
+    if (!f) return -1;
+  }
+  for (int k = 0; k < nsteps2; ++k) {
+    std::mt19937 g = pt.gen;
+    int i = pt.uni_int(g);
+    double u = pt.uni_double(g);
+    pt.step(0);
+    std::vector<accumulator<double>> keep = pt.num_switch;
+    std::unique_ptr<spin_t[]> m = pt.creat_mask(i, u);
+    pt.num_switch = keep;
+    if (pairs) pairs[k] = i;
+    if (us) us[k] = u;
+    if (masks) masks[k] = m[0];
+  }
+  if (spins)
+    for (int j = 0; j < nT; ++j) {
+      long N = sys[j]->get_N();
+      if (kind == 2) std::memcpy(spins + j * N, static_cast<cpu_3d *>(sys[j])->s, N * 8);
+      else std::memcpy(spins + j * N, static_cast<cpu_2dp *>(sys[j])->s, N * 8);
+    }
+  for (cpu_sys *q : sys) delete q;
+  return 0;
+}
+
 // annealing<cpu_sys> with geometric cooling, exactly the reference driver:
 // annealing(T0, sweeps, alfa, sys, out); nsteps x step(stride, PRINT_E).  Writes the
 // driver's text output (one line of 64 energies per measurement) into buf.

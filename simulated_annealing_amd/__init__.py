@@ -34,7 +34,7 @@ EXPORTS = [
     "sa_ladder_set_tuning", "sa_ladder_get_tuning",
     "sa_pt_run", "sa_anneal_run",
     "sa_pt_create", "sa_pt_step", "sa_pt_sync", "sa_pt_steps_done", "sa_pt_result", "sa_pt_ladder",
-    "sa_pt_destroy",
+    "sa_pt_save", "sa_pt_load", "sa_pt_destroy",
     "sa_batch_create", "sa_batch_destroy", "sa_batch_add", "sa_batch_add_pt", "sa_batch_run",
     "sa_batch_sweeps_done", "sa_batch_get_spins", "sa_batch_measure", "sa_batch_pt_result", "sa_batch_bytes",
     "sa_batch_ladder", "sa_batch_remove", "sa_batch_stream", "sa_batch_set_tuning", "sa_batch_get_tuning",
@@ -132,6 +132,8 @@ def lib():
         L.sa_pt_result.argtypes = [_vp, C.c_int64, _vp, _vp, _vp, _vp, _vp]
         L.sa_pt_ladder.argtypes = [_vp]
         L.sa_pt_ladder.restype = _vp
+        L.sa_pt_save.argtypes = [_vp, C.c_char_p]
+        L.sa_pt_load.argtypes = [_vp, C.c_char_p]
         L.sa_pt_destroy.argtypes = [_vp]
         L.sa_batch_create.argtypes = [C.POINTER(_vp)]
         L.sa_batch_destroy.argtypes = [_vp]
@@ -343,6 +345,14 @@ class PT:
 
     def ladder_handle(self):
         return lib().sa_pt_ladder(self.p)
+
+    def save(self, path):
+        """parallel_tempering::save to a file (.tempsys)."""
+        check(lib().sa_pt_save(self.p, str(path).encode()), "sa_pt_save")
+
+    def load(self, path):
+        """parallel_tempering::load from a file (.tempsys); clears the swap log."""
+        check(lib().sa_pt_load(self.p, str(path).encode()), "sa_pt_load")
 
     def result(self, spins=True):
         n = lib().sa_pt_steps_done(self.p)

@@ -3,6 +3,7 @@
 // templates (Python via ctypes, C, other FFIs).
 #include <algorithm>
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <string>
 #include <vector>
@@ -93,6 +94,25 @@ int sa_pt_result(sa_pt *p, int64_t capacity, int32_t *pairs, double *us, uint64_
       if (rc != SA_OK) return rc;
     }
   }
+  return SA_OK;
+}
+
+int sa_pt_save(sa_pt *p, const char *path) {
+  if (!p || !path) return SA_ERR_ARG;
+  std::ofstream f(path, std::ios::binary);
+  if (!f) return SA_ERR_ARG;
+  p->pt->save(f);
+  return f ? SA_OK : SA_ERR_STATE;
+}
+
+int sa_pt_load(sa_pt *p, const char *path) {
+  if (!p || !path) return SA_ERR_ARG;
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return SA_ERR_ARG;
+  p->pt->load(f);
+  if (!f) return SA_ERR_STATE;
+  p->log.clear();
+  p->done = 0;
   return SA_OK;
 }
 

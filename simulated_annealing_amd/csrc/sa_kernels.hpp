@@ -161,6 +161,13 @@ typedef unsigned int sa_v2u32 __attribute__((ext_vector_type(2)));
 typedef unsigned int sa_v4u32 __attribute__((ext_vector_type(4)));
 typedef __amdgpu_buffer_rsrc_t buf_t;
 
+// A buffer access whose per-thread offset is at or beyond the resource size is
+// discarded by the hardware range check: loads return 0 and stores write nothing.
+// Lanes and planes with nothing to load or store use this offset, so every load
+// and store instruction of the z-walk is issued unconditionally and the compiler
+// can count the outstanding loads exactly (no drain of the prefetch).
+constexpr uint32_t kOutOfRange = 0x80000000u;
+
 __device__ __forceinline__ buf_t make_buf(const void *p, uint32_t bytes) {
   // dword 3 = 0x00020000: raw (untyped) dword access on gfx9-family GPUs
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
@@ -328,7 +335,7 @@ struct fused_ctx {
   int64_t L2, N, rank0;
   int par;  // row parity (the template argument P is -1 when it is only known at run time)
   buf_t sin[MB], sout[MB], bjx, bjy, bjz, bw;
-  uint32_t vpair, vpairS, vnext, vring, vrank;  // per-thread byte offsets
+  uint32_t vpair, vpairS, vnext, vring, vrank, vrank1, vstore;  // per-thread byte offsets
   uint32_t plane_bytes;                          // L*L*8
 
   __device__ __forceinline__ uint64_t &c0(int buf, int rr, int m, int kk) const {
@@ -338,17 +345,17 @@ struct fused_ctx {
     return lds[(((buf * (TH + 2) + rr) * MB + m) * 2 + 1) * PR + kk];
   }
 
+  // loads of plane z; no branches (see kOutOfRange): above the top plane the own
+  // pair reads as 0, ring2 is 0 outside the halo rows, g1 is 0 in the halo rows
   __device__ __forceinline__ void load(int z, fused_loads<MB> &q) const {
+    const bool top = z == L - 1;
     const uint32_t P = (uint32_t)z * plane_bytes;
-    const uint32_t Pd = (z == L - 1) ? 0u : P + plane_bytes;
+    const uint32_t Pd = top ? 0u : P + plane_bytes;
+    const uint32_t vn = top ? kOutOfRange : vpair;
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
-      if (z == L - 1) {
-        q.na[m] = q.nb[m] = 0;
-      } else {
-        bld2(sin[m], vpair, Pd, q.na[m], q.nb[m]);
-      }
-      q.ring2[m] = halo ? bld(sin[m], vring, P) : 0;
+      bld2(sin[m], vn, Pd, q.na[m], q.nb[m]);
+      q.ring2[m] = bld(sin[m], vring, P);
     }
     bld2(bjx, vpair, P, q.jxa, q.jxb);
     q.jxn = bld(bjx, vnext, P);
@@ -358,7 +365,7 @@ struct fused_ctx {
     if (!COUNTER) {
       const uint32_t half_plane = (uint32_t)z * (plane_bytes >> 1);
       q.g0 = bld(bw, vrank, half_plane);
-      q.g1 = halo ? 0 : bld(bw, vrank, half_plane + (uint32_t)(N >> 1) * 8u);
+      q.g1 = bld(bw, vrank1, half_plane + (uint32_t)(N >> 1) * 8u);
     }
   }
 
@@ -382,13 +389,19 @@ struct fused_ctx {
     const uint64_t j0w = P ? ld.jxb : ld.jxa, j0e = P ? ld.jxn : ld.jxb;
     const uint64_t j0n = P ? ld.jyb : ld.jya, j0s = P ? ld.jsb : ld.jsa;
     const uint64_t j0d = P ? ld.jdb : ld.jda;
+    // all LDS reads first (the writes below would otherwise order them per system)
+    uint64_t wv[MB], ev[MB], nv[MB], sv[MB];
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      wv[m] = c1(buf, r, m, P ? k : kp);
+      ev[m] = c1(buf, r, m, P ? kn : k);
+      nv[m] = (r == 0) ? ld.ring2[m] : c1(buf, r - 1, m, k);
+      sv[m] = (r == TH + 1) ? ld.ring2[m] : c1(buf, r + 1, m, k);
+    }
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
       const uint64_t si = P ? pb[m] : pa[m];
-      const uint64_t w = c1(buf, r, m, P ? k : kp);
-      const uint64_t e = c1(buf, r, m, P ? kn : k);
-      const uint64_t nn = (r == 0) ? ld.ring2[m] : c1(buf, r - 1, m, k);
-      const uint64_t ss = (r == TH + 1) ? ld.ring2[m] : c1(buf, r + 1, m, k);
+      const uint64_t w = wv[m], e = ev[m], nn = nv[m], ss = sv[m];
       const uint64_t down = last ? first0[m] : (P ? ld.nb[m] : ld.na[m]);
       const count3 c = count6(xor3(j0w, si, w), xor3(j0e, si, e), xor3(j0n, si, nn), xor3(j0s, si, ss),
                               xor3(ju0, si, up0[m]), xor3(j0d, si, down));
@@ -411,26 +424,32 @@ struct fused_ctx {
   }
 
   // colour-1 update of plane q (band rows): reads the new colour-0 words of plane q
-  // from LDS buffer q%3 and stores the pair's new words.
+  // from LDS buffer q%3; oa/ob = the pair's new words (x = 2k, 2k+1).
   template <int PT>
   __device__ __forceinline__ void phase_b(int q, const bstate &bs, uint64_t *up1, uint64_t *first1,
-                                          uint64_t &ju1) const {
+                                          uint64_t &ju1, uint64_t *oa, uint64_t *ob) const {
     const int P = PT >= 0 ? PT : par;
     const int buf = q % 3;
+    uint64_t wv[MB], ev[MB], nv[MB], sv[MB];
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      wv[m] = c0(buf, r, m, P ? kp : k);
+      ev[m] = c0(buf, r, m, P ? k : kn);
+      nv[m] = c0(buf, r - 1, m, k);
+      sv[m] = c0(buf, r + 1, m, k);
+    }
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
       const uint64_t si = bs.si[m];
-      const uint64_t w = c0(buf, r, m, P ? kp : k);
-      const uint64_t e = c0(buf, r, m, P ? k : kn);
-      const uint64_t nn = c0(buf, r - 1, m, k);
-      const uint64_t ss = c0(buf, r + 1, m, k);
+      const uint64_t w = wv[m], e = ev[m], nn = nv[m], ss = sv[m];
       const count3 c = count6(xor3(bs.jw, si, w), xor3(bs.je, si, e), xor3(bs.jn, si, nn), xor3(bs.js, si, ss),
                               xor3(ju1, si, up1[m]), xor3(bs.jd, si, bs.down[m]));
       const sys_table &t = p.tab[sys0 + m];
       const uint64_t n1 = FAST ? update_fast3(t, bs.g, c, si) : update_general(t, 0, bs.g, c, si, 7);
       up1[m] = n1;
       if (q == 0) first1[m] = n1;
-      bst2(sout[m], vpair, (uint32_t)q * plane_bytes, P ? n1 : bs.n0[m], P ? bs.n0[m] : n1);
+      oa[m] = P ? n1 : bs.n0[m];
+      ob[m] = P ? bs.n0[m] : n1;
     }
     ju1 = bs.jd;
   }
@@ -508,8 +527,18 @@ struct fused_ctx {
     __syncthreads();
     // iteration z: near = loads of plane z+1, far = loads of plane z+2 (prefetched)
     auto iteration = [&](int z, fused_loads<MB> &near, fused_loads<MB> &far) {
-      if (z + 2 < L) load(z + 2, far);
-      if (!halo) phase_b<PT>(z, bs, up1, first1, ju1);
+      load(z + 2 < L ? z + 2 : L - 1, far);  // the last two loads are not used
+      {
+        uint64_t oa[MB], ob[MB];
+        if (!halo) {
+          phase_b<PT>(z, bs, up1, first1, ju1, oa, ob);
+        } else {
+#pragma unroll
+          for (int m = 0; m < MB; ++m) oa[m] = ob[m] = 0;
+        }
+#pragma unroll
+        for (int m = 0; m < MB; ++m) bst2(sout[m], vstore, (uint32_t)z * plane_bytes, oa[m], ob[m]);
+      }
       if (z + 1 < L) {
         uint64_t g0, g1;
         words(z + 1, near, x0, x1, blk0, blk1, g0, g1);
@@ -581,8 +610,10 @@ __global__ void __launch_bounds__(640) k_sweep3d_fused(sweep_args p, uint64_t *_
   c.vpair = (uint32_t)(c.row + 2 * c.k) * 8u;
   c.vpairS = (uint32_t)(c.rowS + 2 * c.k) * 8u;
   c.vnext = (uint32_t)(c.row + 2 * c.kn) * 8u;
-  c.vring = (uint32_t)(c.yr2 * L + 2 * c.k + (c.y & 1)) * 8u;
+  c.vring = c.halo ? (uint32_t)(c.yr2 * L + 2 * c.k + (c.y & 1)) * 8u : kOutOfRange;
   c.vrank = (uint32_t)c.rank0 * 8u;
+  c.vrank1 = c.halo ? kOutOfRange : c.vrank;  // halo rows do not update colour 1
+  c.vstore = c.halo ? kOutOfRange : c.vpair;  // nor store anything
   c.par = c.y & 1;
   // The walk contains barriers, so a parity-specialised walk may only be chosen
   // per wave: when a row is a whole number of waves (L/2 a multiple of 64) every

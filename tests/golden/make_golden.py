@@ -2,7 +2,8 @@
 and drivers (oracle/_ref/libref.so, built by oracle/build_ref.sh from the reference
 sources).  Run in the container that has the reference sources:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py                 # ref_vectors.json
+    python tests/golden/make_golden.py --checkpoints   # ckpt_vectors.json + ckpt_*.tempsys
 
 The fixtures are data only: inputs, seeds and the reference's outputs.  Spin arrays
 are stored whole (hex words) for small lattices and as SHA-256 of their
@@ -22,6 +23,7 @@ sys.path.insert(0, ROOT)
 import oracle as O  # noqa: E402
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.json")
+CKPT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ckpt_vectors.json")
 
 
 def hexw(a) -> list[str]:
@@ -138,9 +140,33 @@ def anneal_geometric():
     return out
 
 
+CKPT_FNAME = "sa_pt_%.dat"  # the file-name pattern the PT driver stores (sa_pt_create's default)
+
+
+def checkpoints():
+    """parallel_tempering::save records (.tempsys) written by the reference mid-run,
+    stored as binary files beside this script, plus the run's continuation."""
+    out = []
+    here = os.path.dirname(os.path.abspath(__file__))
+    for kind, L, nT, sweeps, n1, n2, h in [(2, 6, 4, 3, 5, 6, 0.0), (1, 8, 3, 4, 6, 5, 0.25), (0, 10, 4, 2, 4, 7, 0.0)]:
+        T = [0.5 * 4 ** (k / (nT - 1)) for k in range(nT)]
+        name = f"ckpt_k{kind}_L{L}.tempsys"
+        r = O.ref_pt_checkpoint(kind, L, np.array(T), h, 4321, sweeps, n1, n2, CKPT_FNAME, os.path.join(here, name))
+        out.append(dict(file=name, kind=kind, L=L, T=T, h=h, seed=4321, sweeps=sweeps, nsteps1=n1, nsteps2=n2,
+                        fname=CKPT_FNAME, pairs=[int(x) for x in r["pairs"]], us=f64bits(r["us"]),
+                        masks=hexw(r["masks"]), spins_sha=[sha(x) for x in r["spins"]]))
+    return out
+
+
 def main():
     if not O.ref_available():
         O.build_ref()
+    if "--checkpoints" in sys.argv:  # only the checkpoint fixtures
+        with open(CKPT_OUT, "w") as f:
+            json.dump(dict(about="parallel_tempering::save records of the reference (see make_golden.py).",
+                           checkpoints=checkpoints()), f, indent=0, separators=(",", ":"))
+        print("wrote", CKPT_OUT)
+        return
     doc = dict(
         about="Outputs of the reference's compiled CPU classes/drivers (see make_golden.py).",
         **engines(), tables=tables(), sweeps=sweeps(), schedule=temperature_schedule(), pt=pt_runs(),
