@@ -213,7 +213,7 @@ class hip_sys : public gpu_sys {
     uint64_t trailer[4] = {0, (uint64_t)rng_mode_, 0, 0};
     std::memcpy(&trailer[0], "SAHIPX01", 8);
     sa_ladder_get_sweep_index(l(), &trailer[3]);
-    trailer[2] = seed_;
+    sa_ladder_get_seed(l(), &trailer[2]);  // the ladder's key (slots share it)
     os.write(reinterpret_cast<const char *>(trailer), sizeof trailer);
     return os;
   }
@@ -268,12 +268,13 @@ class hip_sys : public gpu_sys {
     apply_T();
     set_spins(s);
     check("load", sa_ladder_set_couplings(l(), j[0].data(), j[1].data(), dim() == 3 ? j[2].data() : nullptr));
-    check("load", sa_ladder_set_engine(l(), eng));
     set_rng((int)trailer[1]);
-    if (trailer[1] != SA_RNG_REPLAY) {
+    if (trailer[1] != SA_RNG_REPLAY) {  // the counter stream's key and position
       seed_ = trailer[2];
+      check("load", sa_ladder_seed(l(), seed_));
       check("load", sa_ladder_set_sweep_index(l(), trailer[3]));
     }
+    check("load", sa_ladder_set_engine(l(), eng));
     return is;
   }
   double get_T() override { return T_; }

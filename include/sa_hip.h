@@ -73,6 +73,8 @@ int sa_ladder_set_rng(sa_ladder *l, int mode);
 /* Re-seed: the host mt19937_64 restarts from `seed` and the counter stream restarts
  * at sweep 0 (set_seed, src/gpu_sys.cuh:18, CPU src/cpu_3d.cpp:170). */
 int sa_ladder_seed(sa_ladder *l, uint64_t seed);
+/* the seed of the last sa_ladder_seed (the counter stream's key) */
+int sa_ladder_get_seed(const sa_ladder *l, uint64_t *seed);
 /* sweeps done since the last seed (the counter-mode sweep index) */
 int sa_ladder_get_sweep_index(const sa_ladder *l, uint64_t *t);
 int sa_ladder_set_sweep_index(sa_ladder *l, uint64_t t);

@@ -25,7 +25,8 @@ TUNE_BATCH_BLOCK, TUNE_BATCH_CHUNK = 101, 102
 EXPORTS = [
     "sa_last_error", "sa_status_string", "sa_version",
     "sa_ladder_create", "sa_ladder_destroy", "sa_ladder_sites", "sa_ladder_nsys",
-    "sa_ladder_set_rng", "sa_ladder_seed", "sa_ladder_get_sweep_index", "sa_ladder_set_sweep_index",
+    "sa_ladder_set_rng", "sa_ladder_seed", "sa_ladder_get_seed", "sa_ladder_get_sweep_index",
+    "sa_ladder_set_sweep_index",
     "sa_ladder_get_engine", "sa_ladder_set_engine",
     "sa_ladder_init_couplings", "sa_ladder_init_spins", "sa_ladder_set_couplings", "sa_ladder_get_couplings",
     "sa_ladder_set_spins", "sa_ladder_get_spins", "sa_ladder_set_temperature", "sa_ladder_sweep",
@@ -98,6 +99,7 @@ def lib():
         L.sa_ladder_nsys.argtypes = [_vp]
         L.sa_ladder_set_rng.argtypes = [_vp, C.c_int]
         L.sa_ladder_seed.argtypes = [_vp, C.c_uint64]
+        L.sa_ladder_get_seed.argtypes = [_vp, C.POINTER(C.c_uint64)]
         L.sa_ladder_get_sweep_index.argtypes = [_vp, C.POINTER(C.c_uint64)]
         L.sa_ladder_set_sweep_index.argtypes = [_vp, C.c_uint64]
         L.sa_ladder_get_engine.argtypes = [_vp, _u64p]

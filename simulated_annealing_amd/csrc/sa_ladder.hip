@@ -314,6 +314,12 @@ int sa_ladder_seed(sa_ladder *l, uint64_t seed) {
   return SA_OK;
 }
 
+int sa_ladder_get_seed(const sa_ladder *l, uint64_t *seed) {
+  if (!l || !seed) return sa_fail(SA_ERR_ARG, "null argument");
+  *seed = l->seed;
+  return SA_OK;
+}
+
 int sa_ladder_get_sweep_index(const sa_ladder *l, uint64_t *t) {
   if (!l || !t) return sa_fail(SA_ERR_ARG, "null argument");
   *t = l->t;
