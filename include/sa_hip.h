@@ -280,12 +280,15 @@ typedef struct sa_serve_config {
   int max_active;     /* requests on the device at once (0: 512) */
   int chunk;          /* sweeps between admission points (0: 10) */
   int digests;        /* 1: read final spins back and digest them */
+  int prep_threads;   /* host threads preparing admitted requests (0: automatic) */
 } sa_serve_config;
 typedef struct sa_serve_stats {
   int64_t submitted, finished, pending;
   int active, peak_active;
   int64_t bytes, peak_bytes, hbm_budget;
-  double busy_s, clock_s;
+  double busy_s, clock_s; /* time from chunk launch to completion; server clock */
+  double admit_s, retire_s; /* host time admitting requests / reading results back */
+  int64_t chunks;
 } sa_serve_stats;
 typedef struct sa_serve sa_serve;
 int sa_serve_create(const sa_serve_config *cfg, sa_serve **out);

@@ -60,13 +60,15 @@ class RequestResult(C.Structure):
 
 
 class ServeConfig(C.Structure):
-    _fields_ = [("hbm_budget", C.c_int64), ("max_active", C.c_int), ("chunk", C.c_int), ("digests", C.c_int)]
+    _fields_ = [("hbm_budget", C.c_int64), ("max_active", C.c_int), ("chunk", C.c_int), ("digests", C.c_int),
+                ("prep_threads", C.c_int)]
 
 
 class ServeStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("finished", C.c_int64), ("pending", C.c_int64), ("active", C.c_int),
                 ("peak_active", C.c_int), ("bytes", C.c_int64), ("peak_bytes", C.c_int64),
-                ("hbm_budget", C.c_int64), ("busy_s", C.c_double), ("clock_s", C.c_double)]
+                ("hbm_budget", C.c_int64), ("busy_s", C.c_double), ("clock_s", C.c_double),
+                ("admit_s", C.c_double), ("retire_s", C.c_double), ("chunks", C.c_int64)]
 
 
 class SAError(RuntimeError):
@@ -544,8 +546,8 @@ def log_digest(pairs, us, masks) -> int:
 class Server:
     """Continuous-batching request loop (sa_serve_* of include/sa_hip.h)."""
 
-    def __init__(self, hbm_budget=0, max_active=0, chunk=0, digests=True):
-        cfg = ServeConfig(hbm_budget, max_active, chunk, 1 if digests else 0)
+    def __init__(self, hbm_budget=0, max_active=0, chunk=0, digests=True, prep_threads=0):
+        cfg = ServeConfig(hbm_budget, max_active, chunk, 1 if digests else 0, prep_threads)
         s = _vp()
         check(lib().sa_serve_create(C.byref(cfg), C.byref(s)), "sa_serve_create")
         self.s = s

@@ -527,8 +527,11 @@ int sync_batch(sa_batch *b) {
   return SA_OK;
 }
 
-int add_instance(sa_batch *b, int kind, int L, int nsys, uint64_t seed, int nsweeps, const double *T, int per_sweep,
-                 double h, pt_state *pt, int *id) {
+// Everything an instance needs on the device, done without touching the batch (so
+// it may run on any thread, while chunks execute): its ladder seeded like main,
+// tables uploaded on the ladder's own stream.
+int prepare_instance(int kind, int L, int nsys, uint64_t seed, int nsweeps, const double *T, int per_sweep, double h,
+                     const pt_state *pt, batch_inst &in) {
   sa_ladder *l = nullptr;
   int rc = sa_ladder_create(kind, L, nsys, &l);
   if (rc) return rc;
@@ -546,14 +549,17 @@ int add_instance(sa_batch *b, int kind, int L, int nsys, uint64_t seed, int nswe
     sa_detail::make_table(kind, T[per_sweep && nsweeps == 0 ? q % nsys : q], h, tabs[q], f, nullptr);
     fast = fast && f;
   }
-  batch_inst in;
+  in = batch_inst();
   in.lad = l;
   in.nsweeps = nsweeps;
   in.per_sweep = per_sweep ? 1 : 0;
   in.fast = fast;
   in.work = (int64_t)L * (L / 2) * nsys;
   if (!rc && hipMalloc(&in.tabs, sizeof(sa::sys_table) * ntab) != hipSuccess) rc = sa_fail(SA_ERR_NOMEM, "tables");
-  if (!rc && hipMemcpy(in.tabs, tabs.data(), sizeof(sa::sys_table) * ntab, hipMemcpyHostToDevice) != hipSuccess)
+  // on the ladder's stream (waited for below): admission may run while a chunk is
+  // on the batch stream, and a null-stream copy would wait for it
+  if (!rc && hipMemcpyAsync(in.tabs, tabs.data(), sizeof(sa::sys_table) * ntab, hipMemcpyHostToDevice,
+                            l->stream) != hipSuccess)
     rc = sa_fail(SA_ERR_HIP, "table upload");
   // fixed temperatures also become the ladder's own tables, for plain sweeps later
   if (!rc && !per_sweep)
@@ -562,21 +568,82 @@ int add_instance(sa_batch *b, int kind, int L, int nsys, uint64_t seed, int nswe
   if (rc) {
     if (in.tabs) (void)hipFree(in.tabs);
     sa_ladder_destroy(l);
+    in = batch_inst();
     return rc;
   }
   in.bytes = sa_detail::batch_instance_bytes(kind, L, nsys, nsweeps, per_sweep);
   if (pt) in.pt = *pt;
+  return SA_OK;
+}
+
+void attach_instance(sa_batch *b, batch_inst &&in, int *id) {
   in.live = true;
   b->bytes_live += in.bytes;
   b->inst.push_back(std::move(in));
   *id = (int)b->inst.size() - 1;
   b->dirty = true;
+}
+
+int add_instance(sa_batch *b, int kind, int L, int nsys, uint64_t seed, int nsweeps, const double *T, int per_sweep,
+                 double h, pt_state *pt, int *id) {
+  batch_inst in;
+  const int rc = prepare_instance(kind, L, nsys, seed, nsweeps, T, per_sweep, h, pt, in);
+  if (rc) return rc;
+  attach_instance(b, std::move(in), id);
   return SA_OK;
+}
+
+// parallel-tempering state as parallel_tempering's constructor sets it up
+pt_state make_pt(int nT, const double *T, double h, int every) {
+  pt_state p;
+  p.every = every;
+  p.T.assign(T, T + nT);
+  std::sort(p.T.begin(), p.T.end());  // parallel_tempering sorts its temperatures
+  p.h = h;
+  p.gen.seed(123);
+  p.uni_b = nT - 2;
+  p.acc_sum.assign(nT - 1, 0.0);
+  p.acc_n.assign(nT - 1, 0);
+  return p;
 }
 
 }  // namespace
 
+struct sa_detail::prepared_instance {
+  batch_inst in;
+};
+
 namespace sa_detail {
+int batch_prepare(int kind, int L, int nsys, uint64_t seed, int nsweeps, const double *T, int per_sweep, double h,
+                  int pt_every, prepared_instance **out) {
+  *out = nullptr;
+  std::unique_ptr<prepared_instance> p(new prepared_instance());
+  int rc;
+  if (pt_every > 0) {
+    if (nsys < 2 || nsweeps % pt_every) return sa_fail(SA_ERR_ARG, "bad parallel-tempering instance");
+    pt_state pt = make_pt(nsys, T, h, pt_every);
+    rc = prepare_instance(kind, L, nsys, seed, nsweeps, pt.T.data(), 0, h, &pt, p->in);
+  } else {
+    rc = prepare_instance(kind, L, nsys, seed, nsweeps, T, per_sweep, h, nullptr, p->in);
+  }
+  if (rc) return rc;
+  *out = p.release();
+  return SA_OK;
+}
+
+int batch_attach(sa_batch *b, prepared_instance *p, int *id) {
+  attach_instance(b, std::move(p->in), id);
+  delete p;
+  return SA_OK;
+}
+
+void batch_discard(prepared_instance *p) {
+  if (!p) return;
+  if (p->in.tabs) (void)hipFree(p->in.tabs);
+  sa_ladder_destroy(p->in.lad);
+  delete p;
+}
+
 int64_t batch_instance_bytes(int kind, int L, int nsys, int nsweeps, int per_sweep) {
   const int ntab = per_sweep ? std::max(1, nsweeps) * nsys : nsys;
   return ladder_bytes(kind, L, nsys) + (int64_t)sizeof(sa::sys_table) * ntab;
@@ -643,6 +710,10 @@ int batch_finish_chunk(sa_batch *b, int k) {
 
 int batch_wait(sa_batch *b) { return sync_batch(b); }
 
+sa_ladder *batch_instance_ladder(sa_batch *b, int id) {
+  return (id >= 0 && id < (int)b->inst.size() && b->inst[id].live) ? b->inst[id].lad : nullptr;
+}
+
 bool batch_done(const sa_batch *b, int id) {
   return id >= 0 && id < (int)b->inst.size() && b->inst[id].live && b->inst[id].t >= b->inst[id].nsweeps;
 }
@@ -658,7 +729,7 @@ int sa_batch_create(sa_batch **out) {
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
       prop.multiProcessorCount > 0)
     b->cus = prop.multiProcessorCount;
-  if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
       hipEventCreate(&b->ev1) != hipSuccess) {
     delete b;
     return sa_fail(SA_ERR_HIP, "stream/event creation failed");
@@ -694,15 +765,7 @@ int sa_batch_add_pt(sa_batch *b, int kind, int L, int nT, const double *T, doubl
                     int nsteps, int *id) {
   if (!b || !T || !id || nT < 2 || every < 1 || nsteps < 0) return sa_fail(SA_ERR_ARG, "bad argument");
   if ((int64_t)every * nsteps > INT32_MAX / 2) return sa_fail(SA_ERR_ARG, "too many sweeps");
-  pt_state p;
-  p.every = every;
-  p.T.assign(T, T + nT);
-  std::sort(p.T.begin(), p.T.end());  // parallel_tempering sorts its temperatures
-  p.h = h;
-  p.gen.seed(123);
-  p.uni_b = nT - 2;
-  p.acc_sum.assign(nT - 1, 0.0);
-  p.acc_n.assign(nT - 1, 0);
+  pt_state p = make_pt(nT, T, h, every);
   return add_instance(b, kind, L, nT, seed, every * nsteps, p.T.data(), 0, h, &p, id);
 }
 

@@ -38,7 +38,7 @@ struct sa_ladder {
   sa::mt19937_64 eng{};
   uint64_t *words_d = nullptr, *words_h = nullptr;
   int64_t words_cap = 0;  // sweeps that fit in the staging buffers
-  int *cnt_h = nullptr;
+  std::vector<int> cnt_h;  // host copy of the measurement counters
   int *sys_list = nullptr;  // device copy of a measured subset
   int sys_per_thread = 0;  // SA_TUNE_SYS_PER_THREAD, 0 = automatic
   int sweep3d_kernel = 0;  // SA_TUNE_SWEEP3D_KERNEL, 0 = automatic
@@ -63,4 +63,16 @@ int batch_launch_chunk(sa_batch *b, int max_sweeps, int *k);
 int batch_finish_chunk(sa_batch *b, int k);
 int batch_wait(sa_batch *b);
 bool batch_done(const sa_batch *b, int id);
+// An instance prepared off the batch (ladder seeded like main, tables uploaded),
+// from any thread; attaching it (loop thread) makes it part of the next chunk.
+// pt_every > 0: a parallel-tempering instance of nsys temperatures T exchanging
+// every pt_every sweeps (nsweeps a multiple of it).
+struct prepared_instance;
+int batch_prepare(int kind, int L, int nsys, uint64_t seed, int nsweeps, const double *T, int per_sweep, double h,
+                  int pt_every, prepared_instance **out);
+int batch_attach(sa_batch *b, prepared_instance *p, int *id);
+void batch_discard(prepared_instance *p);
+// an instance's ladder without waiting for the batch (only for an instance that
+// has finished: running chunks skip it and never touch its memory)
+sa_ladder *batch_instance_ladder(sa_batch *b, int id);
 }  // namespace sa_detail

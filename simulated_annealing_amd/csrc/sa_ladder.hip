@@ -43,7 +43,6 @@ void free_all(sa_ladder *l) {
   if (l->s_alt) (void)hipFree(l->s_alt);
   if (l->words_d) (void)hipFree(l->words_d);
   if (l->words_h) (void)hipHostFree(l->words_h);
-  if (l->cnt_h) (void)hipHostFree(l->cnt_h);
   if (l->sys_list) (void)hipFree(l->sys_list);
   if (l->stream) (void)hipStreamDestroy(l->stream);
 }
@@ -137,7 +136,7 @@ fused_cfg pick_fused(const sa_ladder *l) {
       c.mb = mb;
       const int ntile = l->L / th;
       const bool can = ntile % 8 == 0;
-      c.xcd = (l->xcd_map < 0) ? 0 : (l->xcd_map && can);
+      c.xcd = (l->xcd_map < 0) ? can : (l->xcd_map && can);  // measured faster where it applies
       return c;
     }
   }
@@ -257,12 +256,11 @@ int sa_ladder_create(int kind, int L, int nsys, sa_ladder **out) {
                                     : sa_fail(SA_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
     return true;
   };
-  if (bad(hipGetDevice(&l->device), "hipGetDevice") || bad(hipStreamCreate(&l->stream), "hipStreamCreate") ||
+  if (bad(hipGetDevice(&l->device), "hipGetDevice") || bad(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking), "hipStreamCreate") ||
       bad(hipMalloc(&l->s, (size_t)nsys * N * 8), "spins") || bad(hipMalloc(&l->jx, N * 8), "jx") ||
       bad(hipMalloc(&l->jy, N * 8), "jy") || bad(hipMalloc(&l->jz, N * 8), "jz") ||
       bad(hipMalloc(&l->tab, sizeof(sa::sys_table) * nsys), "tables") ||
       bad(hipMalloc(&l->cnt, sizeof(int) * 128 * nsys), "counters") ||
-      bad(hipHostMalloc(&l->cnt_h, sizeof(int) * 128 * nsys, hipHostMallocDefault), "pinned counters") ||
       bad(hipMalloc(&l->sys_list, sizeof(int) * nsys), "system list") ||
       bad(hipMemsetAsync(l->s, 0, (size_t)nsys * N * 8, l->stream), "memset") ||
       bad(hipMemsetAsync(l->jx, 0, N * 8, l->stream), "memset") ||
@@ -274,6 +272,7 @@ int sa_ladder_create(int kind, int L, int nsys, sa_ladder **out) {
   }
   l->htab.assign(nsys, sa::sys_table{});
   l->fast.assign(nsys, 0);
+  l->cnt_h.assign((size_t)128 * nsys, 0);
   l->eng.seed(5489u);
   *out = l;
   for (int i = 0; i < nsys; ++i) {
@@ -514,7 +513,9 @@ int measure_list(sa_ladder *l, const int *list, int n) {
     hipLaunchKernelGGL(sa::k_measure<sa::KIND_2P>, grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, l->L,
                        l->N, n, dlist, l->cnt);
   SA_HIP_TRY(hipGetLastError());
-  SA_HIP_TRY(hipMemcpyAsync(l->cnt_h, l->cnt, sizeof(int) * cnt_n, hipMemcpyDeviceToHost, l->stream));
+  // pageable destination: the copy completes before the synchronisation below, and
+  // creating a ladder (e.g. per served request) then needs no pinned allocation
+  SA_HIP_TRY(hipMemcpyAsync(l->cnt_h.data(), l->cnt, sizeof(int) * cnt_n, hipMemcpyDeviceToHost, l->stream));
   SA_HIP_TRY(hipStreamSynchronize(l->stream));
   return SA_OK;
 }

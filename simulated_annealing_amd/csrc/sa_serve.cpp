@@ -9,7 +9,9 @@
 // lattice, couplings and random stream stay on the device from admission to
 // completion.  After each chunk the finished requests are retired (digest of the
 // final spins, final energies, memory released) and their latency recorded.
-// Admission work (seeding, coupling draws, uploads) is done while the chunk runs.
+// Admission work (seeding, coupling draws, uploads) runs on a pool of preparation
+// threads; the loop thread only attaches prepared instances between chunks, and
+// reads finished requests back while the next chunk runs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,6 +21,7 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "sa_hip.h"
@@ -72,6 +75,19 @@ int check_request(const sa_request &r) {
 struct sa_serve {
   sa_serve_config cfg{};
   sa_batch *b = nullptr;
+  int device = 0;
+  // preparation pool: jobs = admitted request ids, ready = prepared instances
+  std::vector<std::thread> workers;
+  std::deque<int64_t> jobs;
+  struct ready_item {
+    int64_t id;
+    int rc;
+    sa_detail::prepared_instance *p;
+  };
+  std::vector<ready_item> ready;
+  std::condition_variable cv_jobs;
+  bool stopping = false;
+  int preparing = 0;  // admitted, not attached yet
   clock_type::time_point t0;
   std::mutex mu;
   std::condition_variable cv;
@@ -81,7 +97,10 @@ struct sa_serve {
   int64_t finished = 0;
   int64_t bytes = 0, peak_bytes = 0;
   int peak_active = 0;
-  double busy_s = 0;  // wall time with requests on the device
+  double busy_s = 0;    // wall time from chunk launch to chunk completion
+  double admit_s = 0;   // host time admitting requests
+  double retire_s = 0;  // host time reading results back
+  int64_t chunks = 0;
 
   double now() const { return std::chrono::duration<double>(clock_type::now() - t0).count(); }
 };
@@ -94,26 +113,41 @@ req_state &req(sa_serve *s, int64_t id) {
   return s->reqs[id];
 }
 
-int admit(sa_serve *s, int64_t id) {
-  req_state &q = req(s, id);
-  const sa_request &r = q.r;
-  std::vector<double> T;
+// the device state of a request, on a preparation thread
+int prepare_request(const sa_request &r, sa_detail::prepared_instance **out) {
   int rc = sa_request_temperatures(&r, nullptr, 0);
   if (rc < 0) return rc;
-  T.resize(rc);
+  std::vector<double> T(rc);
   sa_request_temperatures(&r, T.data(), (int64_t)T.size());
-  int inst = -1;
   if (r.mode == SA_REQ_PT)
-    rc = sa_batch_add_pt(s->b, r.kind, r.L, r.nT, T.data(), r.h, r.seed, r.every, r.nsweeps / r.every, &inst);
-  else
-    rc = sa_batch_add(s->b, r.kind, r.L, 1, r.seed, r.nsweeps, T.data(), 1, r.h, &inst);
-  if (rc) return rc;
-  q.inst = inst;
-  return SA_OK;
+    return sa_detail::batch_prepare(r.kind, r.L, r.nT, r.seed, r.nsweeps, T.data(), 0, r.h, r.every, out);
+  return sa_detail::batch_prepare(r.kind, r.L, 1, r.seed, r.nsweeps, T.data(), 1, r.h, 0, out);
 }
 
-void finish_request(sa_serve *s, int64_t id, int status) {
-  std::lock_guard<std::mutex> g(s->mu);
+void worker_main(sa_serve *s) {
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    int64_t id;
+    sa_request r;
+    {
+      std::unique_lock<std::mutex> g(s->mu);
+      s->cv_jobs.wait(g, [&] { return s->stopping || !s->jobs.empty(); });
+      if (s->jobs.empty()) return;  // stopping
+      id = s->jobs.front();
+      s->jobs.pop_front();
+      r = s->reqs[id].r;
+    }
+    sa_detail::prepared_instance *p = nullptr;
+    const int rc = prepare_request(r, &p);
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      s->ready.push_back({id, rc, p});
+    }
+    s->cv.notify_all();
+  }
+}
+
+void finish_locked(sa_serve *s, int64_t id, int status) {
   req_state &q = s->reqs[id];
   q.res.status = status;
   q.res.finished = s->now();
@@ -121,11 +155,18 @@ void finish_request(sa_serve *s, int64_t id, int status) {
   s->finished += 1;
 }
 
-// final results of a finished request, then its device memory is released
-int retire(sa_serve *s, int64_t id) {
+void finish_request(sa_serve *s, int64_t id, int status) {
+  std::lock_guard<std::mutex> g(s->mu);
+  finish_locked(s, id, status);
+}
+
+// final results of a finished request.  Runs while the next chunk is on the GPU:
+// the instance is finished, so the chunk skips it, and everything here uses the
+// instance's own stream (measure, spin copies).
+int read_results(sa_serve *s, int64_t id) {
   req_state &q = req(s, id);
   sa_request_result &res = q.res;
-  sa_ladder *l = sa_batch_ladder(s->b, q.inst);
+  sa_ladder *l = sa_detail::batch_instance_ladder(s->b, q.inst);
   if (!l) return SA_ERR_STATE;
   const int nsys = sa_ladder_nsys(l);
   std::vector<int32_t> E((size_t)nsys * 64), M((size_t)nsys * 64);
@@ -163,46 +204,70 @@ int retire(sa_serve *s, int64_t id) {
     res.log_digest = h;
     res.steps = n;
   }
-  rc = sa_batch_remove(s->b, q.inst);
-  if (rc) return rc;
-  s->bytes -= q.bytes;
-  q.inst = -1;
   return SA_OK;
 }
 
-// admit arrived requests in arrival order while memory and slots allow
-void admit_arrived(sa_serve *s) {
-  for (;;) {
-    int64_t id;
-    {
-      std::lock_guard<std::mutex> g(s->mu);
-      if (s->pending.empty()) return;
-      id = s->pending.front();
+// release a retired request's device memory (waits for the batch stream)
+int release(sa_serve *s, int64_t id) {
+  req_state &q = req(s, id);
+  const int rc = sa_batch_remove(s->b, q.inst);
+  s->bytes -= q.bytes;
+  q.inst = -1;
+  return rc;
+}
+
+// admit arrived requests in arrival order while memory and slots allow: reserve
+// their memory and hand them to the preparation threads
+void dispatch_arrived(sa_serve *s) {
+  bool any = false;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    while (!s->pending.empty()) {
+      const int64_t id = s->pending.front();
       req_state &q = s->reqs[id];
-      if (q.r.arrival > s->now()) return;
-      if (q.bytes <= s->cfg.hbm_budget &&
-          ((int)s->active.size() >= s->cfg.max_active || s->bytes + q.bytes > s->cfg.hbm_budget))
-        return;  // wait for memory / a slot (arrival order is kept)
+      if (q.r.arrival > s->now()) break;
+      if (q.bytes > s->cfg.hbm_budget) {  // can never fit
+        s->pending.pop_front();
+        sa_fail(SA_ERR_NOMEM, "request %lld needs %lld bytes, over the budget", (long long)id, (long long)q.bytes);
+        q.res.admitted = s->now();
+        finish_locked(s, id, SA_ERR_NOMEM);
+        continue;
+      }
+      if ((int)s->active.size() + s->preparing >= s->cfg.max_active || s->bytes + q.bytes > s->cfg.hbm_budget)
+        break;  // wait for memory / a slot (arrival order is kept)
       s->pending.pop_front();
-    }
-    req_state &q = req(s, id);
-    if (q.bytes > s->cfg.hbm_budget) {
-      sa_fail(SA_ERR_NOMEM, "request %lld needs %lld bytes, over the budget", (long long)id, (long long)q.bytes);
-      finish_request(s, id, SA_ERR_NOMEM);
-      continue;
-    }
-    {
-      std::lock_guard<std::mutex> g(s->mu);
       q.res.admitted = s->now();
+      s->bytes += q.bytes;
+      s->peak_bytes = std::max(s->peak_bytes, s->bytes);
+      s->preparing += 1;
+      s->jobs.push_back(id);
+      any = true;
     }
-    const int rc = admit(s, id);
+  }
+  if (any) s->cv_jobs.notify_all();
+}
+
+// prepared instances join the batch (loop thread, between chunks)
+void attach_ready(sa_serve *s) {
+  std::vector<sa_serve::ready_item> got;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    got.swap(s->ready);
+  }
+  for (auto &it : got) {
+    req_state &q = req(s, it.id);
+    int rc = it.rc;
+    int inst = -1;
+    if (!rc) rc = sa_detail::batch_attach(s->b, it.p, &inst);
+    std::lock_guard<std::mutex> g(s->mu);
+    s->preparing -= 1;
     if (rc) {
-      finish_request(s, id, rc);
+      s->bytes -= q.bytes;
+      finish_locked(s, it.id, rc);
       continue;
     }
-    s->active.push_back(id);
-    s->bytes += q.bytes;
-    s->peak_bytes = std::max(s->peak_bytes, s->bytes);
+    q.inst = inst;
+    s->active.push_back(it.id);
     s->peak_active = std::max(s->peak_active, (int)s->active.size());
   }
 }
@@ -260,6 +325,10 @@ int sa_serve_create(const sa_serve_config *cfg, sa_serve **out) {
     return rc;
   }
   sa_batch_set_tuning(s->b, SA_TUNE_BATCH_CHUNK, s->cfg.chunk);
+  (void)hipGetDevice(&s->device);
+  int nw = s->cfg.prep_threads;
+  if (nw <= 0) nw = (int)std::min<unsigned>(8, std::max(1u, std::thread::hardware_concurrency() / 2));
+  for (int i = 0; i < nw; ++i) s->workers.emplace_back(worker_main, s);
   s->t0 = clock_type::now();
   *out = s;
   return SA_OK;
@@ -267,6 +336,14 @@ int sa_serve_create(const sa_serve_config *cfg, sa_serve **out) {
 
 int sa_serve_destroy(sa_serve *s) {
   if (!s) return SA_OK;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stopping = true;
+    s->jobs.clear();
+  }
+  s->cv_jobs.notify_all();
+  for (auto &t : s->workers) t.join();
+  for (auto &it : s->ready) sa_detail::batch_discard(it.p);
   sa_batch_destroy(s->b);
   delete s;
   return SA_OK;
@@ -307,48 +384,62 @@ int sa_serve_submit(sa_serve *s, const sa_request *r, int64_t *id) {
 
 int sa_serve_run(sa_serve *s, int64_t n_finished) {
   if (!s) return sa_fail(SA_ERR_ARG, "null server");
+  std::vector<int64_t> done_dev;  // finished on the device, results not read yet
+  std::vector<int64_t> to_free;   // results read, memory not released yet
   for (;;) {
+    if (!to_free.empty()) {  // the batch stream is idle here: releasing costs no overlap
+      for (int64_t id : to_free) {
+        const int rc = release(s, id);
+        if (rc) return rc;
+      }
+      to_free.clear();
+    }
+    double h0 = s->now();
+    dispatch_arrived(s);
+    attach_ready(s);
+    s->admit_s += s->now() - h0;
     {
       std::unique_lock<std::mutex> g(s->mu);
       const int64_t target = n_finished > 0 ? n_finished : (int64_t)s->reqs.size();
-      if (s->finished >= target) return SA_OK;
-      if (s->active.empty()) {
-        // idle: wait for the next arrival (or a submission)
-        if (s->pending.empty()) {
-          if (n_finished <= 0) return SA_OK;
-          s->cv.wait_for(g, std::chrono::milliseconds(5));
-          continue;
-        }
-        const double wait = s->reqs[s->pending.front()].r.arrival - s->now();
-        if (wait > 0) {
-          s->cv.wait_for(g, std::chrono::duration<double>(std::min(wait, 0.005)));
-          continue;
-        }
+      if (s->finished >= target && done_dev.empty()) return SA_OK;
+      if (s->active.empty() && done_dev.empty()) {
+        // idle: wait for a prepared instance, the next arrival or a submission
+        if (s->pending.empty() && s->preparing == 0 && s->ready.empty() && n_finished <= 0) return SA_OK;
+        double wait = 0.005;
+        if (!s->pending.empty()) wait = std::min(wait, s->reqs[s->pending.front()].r.arrival - s->now());
+        if (s->ready.empty() && wait > 0) s->cv.wait_for(g, std::chrono::duration<double>(wait));
+        continue;
       }
     }
-    admit_arrived(s);
-    if (s->active.empty()) continue;
-    const double c0 = s->now();
     int k = 0;
-    int rc = sa_detail::batch_launch_chunk(s->b, s->cfg.chunk, &k);
-    if (rc) return rc;
-    admit_arrived(s);  // host work of new arrivals overlaps the chunk
-    if (k > 0) {
-      rc = sa_detail::batch_finish_chunk(s->b, k);
+    const double c0 = s->now();
+    if (!s->active.empty()) {
+      const int rc = sa_detail::batch_launch_chunk(s->b, s->cfg.chunk, &k);
       if (rc) return rc;
     }
-    rc = sa_detail::batch_wait(s->b);
-    if (rc) return rc;
-    s->busy_s += s->now() - c0;
-    // retire finished requests
+    // host work that overlaps the chunk: results of the requests that finished in
+    // the previous chunk, then admission of new arrivals
+    h0 = s->now();
+    for (int64_t id : done_dev) {
+      const int rc = read_results(s, id);
+      finish_request(s, id, rc);
+      to_free.push_back(id);
+    }
+    done_dev.clear();
+    s->retire_s += s->now() - h0;
+    dispatch_arrived(s);
+    if (k > 0) {
+      int rc = sa_detail::batch_finish_chunk(s->b, k);
+      if (rc) return rc;
+      rc = sa_detail::batch_wait(s->b);
+      if (rc) return rc;
+      s->busy_s += s->now() - c0;
+      s->chunks += 1;
+    }
     std::vector<int64_t> still;
     for (int64_t id : s->active) {
-      if (sa_detail::batch_done(s->b, req(s, id).inst)) {
-        rc = retire(s, id);  // the result is complete once read back
-        finish_request(s, id, rc);
-      } else {
-        still.push_back(id);
-      }
+      if (sa_detail::batch_done(s->b, req(s, id).inst)) done_dev.push_back(id);
+      else still.push_back(id);
     }
     s->active.swap(still);
   }
@@ -379,6 +470,9 @@ int sa_serve_get_stats(sa_serve *s, sa_serve_stats *st) {
   st->peak_bytes = s->peak_bytes;
   st->hbm_budget = s->cfg.hbm_budget;
   st->busy_s = s->busy_s;
+  st->admit_s = s->admit_s;
+  st->retire_s = s->retire_s;
+  st->chunks = s->chunks;
   st->clock_s = s->now();
   return SA_OK;
 }
