@@ -139,14 +139,18 @@ __device__ __forceinline__ uint64_t update_general(const sys_table &t, int tb0, 
 // applied to both halves of the word.
 //   3D (levels 4,5,6 thresholded): flip = ~c2 | (c1 ? a6 : (c0 ? a5 : a4))
 //   2D (levels 3,4):               flip = c2 ? a4 : ((c1 & c0) ? a3 : ~0)
-__device__ __forceinline__ uint64_t update_fast3(const sys_table &t, uint64_t g, const count3 &c, uint64_t si) {
-  const uint32_t a4 = (g < t.thr[0][4]) ? ~0u : 0u, a5 = (g < t.thr[0][5]) ? ~0u : 0u;
-  const uint32_t a6 = (g < t.thr[0][6]) ? ~0u : 0u;
+__device__ __forceinline__ uint64_t update_fast3t(uint64_t t4, uint64_t t5, uint64_t t6, uint64_t g, const count3 &c,
+                                                  uint64_t si) {
+  const uint32_t a4 = (g < t4) ? ~0u : 0u, a5 = (g < t5) ? ~0u : 0u;
+  const uint32_t a6 = (g < t6) ? ~0u : 0u;
   const uint32_t vlo = b3<B3_MUX>((uint32_t)c.b1, a6, b3<B3_MUX>((uint32_t)c.b0, a5, a4));
   const uint32_t vhi = b3<B3_MUX>((uint32_t)(c.b1 >> 32), a6, b3<B3_MUX>((uint32_t)(c.b0 >> 32), a5, a4));
   const uint32_t lo = b3<B3_FLIP>((uint32_t)c.b2, vlo, (uint32_t)si);
   const uint32_t hi = b3<B3_FLIP>((uint32_t)(c.b2 >> 32), vhi, (uint32_t)(si >> 32));
   return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t update_fast3(const sys_table &t, uint64_t g, const count3 &c, uint64_t si) {
+  return update_fast3t(t.thr[0][4], t.thr[0][5], t.thr[0][6], g, c, si);
 }
 __device__ __forceinline__ uint64_t update_fast2(const sys_table &t, uint64_t g, const count3 &c, uint64_t si) {
   const uint64_t a3 = lvl(g < t.thr[0][3]), a4 = lvl(g < t.thr[0][4]);
@@ -330,7 +334,8 @@ template <bool COUNTER, bool FAST, int MB, int TH>
 struct fused_ctx {
   const sweep_args &p;
   uint64_t *lds;
-  int L, PR, r, k, kn, kp, y, row, rowS, yr2, sys0;
+  int L, PR, r, rn, rs, k, kn, kp, y, row, rowS, yr2, sys0;
+  uint64_t t4[MB], t5[MB], t6[MB];  // FAST: the thresholded levels of each system's table
   bool halo;
   int64_t L2, N, rank0;
   int par;  // row parity (the template argument P is -1 when it is only known at run time)
@@ -376,37 +381,63 @@ struct fused_ctx {
     uint64_t si[MB], down[MB], n0[MB];
   };
 
-  // colour-0 update of plane q (all rows, halo included).  q_loads holds the loads
-  // of plane q, pair[] the old pair at plane q.  Writes the new colour-0 words into
-  // LDS buffer q%3 and fills bs for the colour-1 update of plane q.
+  // in-plane neighbour words of the thread's site of one colour, per system
+  struct nbrs {
+    uint64_t w[MB], e[MB], n[MB], s[MB];
+  };
+  // Both phases' LDS reads are issued together at the top of an iteration (they
+  // read different plane buffers than the iteration writes), so a wave waits for
+  // LDS once per plane.  Rows rn/rs are r-1/r+1 clamped to the band: the halo
+  // rows take ring2 instead, or do not use the values (phase B).
   template <int PT>
-  __device__ __forceinline__ void phase_a(int q, const fused_loads<MB> &ld, const uint64_t *pa, const uint64_t *pb,
-                                          uint64_t g0, uint64_t g1, uint64_t *up0, uint64_t *first0,
-                                          const uint64_t *first1, uint64_t &ju0, bstate &bs) const {
+  __device__ __forceinline__ void read_a(int q, const fused_loads<MB> &ld, nbrs &v) const {
+    const int P = PT >= 0 ? PT : par;
+    const int buf = q % 3;
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      v.w[m] = c1(buf, r, m, P ? k : kp);
+      v.e[m] = c1(buf, r, m, P ? kn : k);
+      const uint64_t nn = c1(buf, rn, m, k), ss = c1(buf, rs, m, k);
+      v.n[m] = (r == 0) ? ld.ring2[m] : nn;
+      v.s[m] = (r == TH + 1) ? ld.ring2[m] : ss;
+    }
+  }
+  template <int PT>
+  __device__ __forceinline__ void read_b(int q, nbrs &v) const {
+    const int P = PT >= 0 ? PT : par;
+    const int buf = q % 3;
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      v.w[m] = c0(buf, r, m, P ? kp : k);
+      v.e[m] = c0(buf, r, m, P ? k : kn);
+      v.n[m] = c0(buf, rn, m, k);
+      v.s[m] = c0(buf, rs, m, k);
+    }
+  }
+
+  // colour-0 update of plane q (all rows, halo included).  q_loads holds the loads
+  // of plane q, pair[] the old pair at plane q, v the colour-1 neighbours.  Writes
+  // the new colour-0 words into LDS buffer q%3 and fills bs for the colour-1 update
+  // of plane q.
+  template <int PT>
+  __device__ __forceinline__ void phase_a(int q, const fused_loads<MB> &ld, const nbrs &v, const uint64_t *pa,
+                                          const uint64_t *pb, uint64_t g0, uint64_t g1, uint64_t *up0,
+                                          uint64_t *first0, const uint64_t *first1, uint64_t &ju0, bstate &bs) const {
     const int P = PT >= 0 ? PT : par;
     const int buf = q % 3;
     const bool last = q == L - 1;
     const uint64_t j0w = P ? ld.jxb : ld.jxa, j0e = P ? ld.jxn : ld.jxb;
     const uint64_t j0n = P ? ld.jyb : ld.jya, j0s = P ? ld.jsb : ld.jsa;
     const uint64_t j0d = P ? ld.jdb : ld.jda;
-    // all LDS reads first (the writes below would otherwise order them per system)
-    uint64_t wv[MB], ev[MB], nv[MB], sv[MB];
-#pragma unroll
-    for (int m = 0; m < MB; ++m) {
-      wv[m] = c1(buf, r, m, P ? k : kp);
-      ev[m] = c1(buf, r, m, P ? kn : k);
-      nv[m] = (r == 0) ? ld.ring2[m] : c1(buf, r - 1, m, k);
-      sv[m] = (r == TH + 1) ? ld.ring2[m] : c1(buf, r + 1, m, k);
-    }
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
       const uint64_t si = P ? pb[m] : pa[m];
-      const uint64_t w = wv[m], e = ev[m], nn = nv[m], ss = sv[m];
+      const uint64_t w = v.w[m], e = v.e[m], nn = v.n[m], ss = v.s[m];
       const uint64_t down = last ? first0[m] : (P ? ld.nb[m] : ld.na[m]);
       const count3 c = count6(xor3(j0w, si, w), xor3(j0e, si, e), xor3(j0n, si, nn), xor3(j0s, si, ss),
                               xor3(ju0, si, up0[m]), xor3(j0d, si, down));
-      const sys_table &t = p.tab[sys0 + m];
-      const uint64_t n0 = FAST ? update_fast3(t, g0, c, si) : update_general(t, 0, g0, c, si, 7);
+      const uint64_t n0 =
+          FAST ? update_fast3t(t4[m], t5[m], t6[m], g0, c, si) : update_general(p.tab[sys0 + m], 0, g0, c, si, 7);
       c0(buf, r, m, k) = n0;
       up0[m] = n0;
       if (q == 0) first0[m] = n0;
@@ -426,26 +457,17 @@ struct fused_ctx {
   // colour-1 update of plane q (band rows): reads the new colour-0 words of plane q
   // from LDS buffer q%3; oa/ob = the pair's new words (x = 2k, 2k+1).
   template <int PT>
-  __device__ __forceinline__ void phase_b(int q, const bstate &bs, uint64_t *up1, uint64_t *first1,
+  __device__ __forceinline__ void phase_b(int q, const nbrs &v, const bstate &bs, uint64_t *up1, uint64_t *first1,
                                           uint64_t &ju1, uint64_t *oa, uint64_t *ob) const {
     const int P = PT >= 0 ? PT : par;
-    const int buf = q % 3;
-    uint64_t wv[MB], ev[MB], nv[MB], sv[MB];
-#pragma unroll
-    for (int m = 0; m < MB; ++m) {
-      wv[m] = c0(buf, r, m, P ? kp : k);
-      ev[m] = c0(buf, r, m, P ? k : kn);
-      nv[m] = c0(buf, r - 1, m, k);
-      sv[m] = c0(buf, r + 1, m, k);
-    }
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
       const uint64_t si = bs.si[m];
-      const uint64_t w = wv[m], e = ev[m], nn = nv[m], ss = sv[m];
+      const uint64_t w = v.w[m], e = v.e[m], nn = v.n[m], ss = v.s[m];
       const count3 c = count6(xor3(bs.jw, si, w), xor3(bs.je, si, e), xor3(bs.jn, si, nn), xor3(bs.js, si, ss),
                               xor3(ju1, si, up1[m]), xor3(bs.jd, si, bs.down[m]));
-      const sys_table &t = p.tab[sys0 + m];
-      const uint64_t n1 = FAST ? update_fast3(t, bs.g, c, si) : update_general(t, 0, bs.g, c, si, 7);
+      const uint64_t n1 = FAST ? update_fast3t(t4[m], t5[m], t6[m], bs.g, c, si)
+                               : update_general(p.tab[sys0 + m], 0, bs.g, c, si, 7);
       up1[m] = n1;
       if (q == 0) first1[m] = n1;
       oa[m] = P ? n1 : bs.n0[m];
@@ -516,8 +538,10 @@ struct fused_ctx {
     __syncthreads();
     {
       uint64_t g0, g1;
+      nbrs va;
+      read_a<PT>(0, A, va);
       words(0, A, x0, x1, blk0, blk1, g0, g1);
-      phase_a<PT>(0, A, pa, pb, g0, g1, up0, first0, first1, ju0, bs);
+      phase_a<PT>(0, A, va, pa, pb, g0, g1, up0, first0, first1, ju0, bs);
     }
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
@@ -528,10 +552,12 @@ struct fused_ctx {
     // iteration z: near = loads of plane z+1, far = loads of plane z+2 (prefetched)
     auto iteration = [&](int z, fused_loads<MB> &near, fused_loads<MB> &far) {
       load(z + 2 < L ? z + 2 : L - 1, far);  // the last two loads are not used
+      nbrs vb;
+      read_b<PT>(z, vb);
       {
         uint64_t oa[MB], ob[MB];
         if (!halo) {
-          phase_b<PT>(z, bs, up1, first1, ju1, oa, ob);
+          phase_b<PT>(z, vb, bs, up1, first1, ju1, oa, ob);
         } else {
 #pragma unroll
           for (int m = 0; m < MB; ++m) oa[m] = ob[m] = 0;
@@ -541,8 +567,10 @@ struct fused_ctx {
       }
       if (z + 1 < L) {
         uint64_t g0, g1;
+        nbrs va;
+        read_a<PT>(z + 1, near, va);
         words(z + 1, near, x0, x1, blk0, blk1, g0, g1);
-        phase_a<PT>(z + 1, near, pa, pb, g0, g1, up0, first0, first1, ju0, bs);
+        phase_a<PT>(z + 1, near, va, pa, pb, g0, g1, up0, first0, first1, ju0, bs);
         if (z + 2 < L) put_old<PT>(z + 2, near.na, near.nb);
 #pragma unroll
         for (int m = 0; m < MB; ++m) {
@@ -587,12 +615,21 @@ __global__ void __launch_bounds__(640) k_sweep3d_fused(sweep_args p, uint64_t *_
   c.kn = c.k == PR - 1 ? 0 : c.k + 1;
   c.kp = c.k == 0 ? PR - 1 : c.k - 1;
   c.halo = (c.r == 0) || (c.r == TH + 1);
+  c.rn = c.r == 0 ? 0 : c.r - 1;
+  c.rs = c.r == TH + 1 ? TH + 1 : c.r + 1;
   c.y = (tile * TH - 1 + c.r + L) % L;
   const int ym = c.y == 0 ? L - 1 : c.y - 1, yp = c.y == L - 1 ? 0 : c.y + 1;
   c.row = c.y * L;
   c.rowS = yp * L;
   c.yr2 = (c.r == 0) ? ym : yp;  // halo rows: the row beyond the halo
   c.sys0 = sb * MB;
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {  // loaded once for the whole walk
+    const sys_table &t = p.tab[c.sys0 + m];
+    c.t4[m] = t.thr[0][4];
+    c.t5[m] = t.thr[0][5];
+    c.t6[m] = t.thr[0][6];
+  }
   c.L2 = (int64_t)L * L;
   c.N = p.N;
   c.rank0 = (int64_t)c.y * PR + c.k;  // visit rank inside a plane half

@@ -114,6 +114,8 @@ size_t fused_lds_bytes(int L, int th, int mb) { return (size_t)3 * (th + 2) * mb
 bool fused_ok(const sa_ladder *l, int th, int mb) {
   if (l->kind != SA_KIND_3D || l->L % th != 0 || th > l->L) return false;
   if (l->nsys % mb != 0) return false;
+  // buffer byte offsets (and the out-of-range marker 2^31 above them) are 32-bit
+  if ((int64_t)l->N * 8 >= (int64_t)1 << 31) return false;
   if ((th + 2) * (l->L / 2) > 640) return false;  // the kernel's launch bound
   return fused_lds_bytes(l->L, th, mb) <= 64 * 1024;
 }
