@@ -156,7 +156,11 @@ def serving(sa, n=1000, rate=40.0, budget=0, max_active=0):
     for w in W:
         srv.submit(sa.make_request(w.kind, w.L, w.mode, w.nsweeps, w.seed, nT=w.nT, every=10,
                                    arrival=base + w.arrival))
-    srv.run()
+    # run in segments so a long stream reports progress (the loop keeps its clock)
+    seg = max(1, n // 20)
+    for m in list(range(seg, n, seg)) + [n]:
+        srv.run(m)
+        print(f"[serve] {m}/{n} finished at {srv.clock():.1f} s", file=sys.stderr, flush=True)
     res = srv.results()
     st = srv.stats()
     srv.close()
@@ -251,9 +255,13 @@ def main():
             prof = json.load(f)
         if prof.get("kernel") == roofline_kernel(fused):
             traffic = prof.get("hbm_bytes_per_launch")
+    # the same launches on the single-lattice basis (40 B per word-sweep: couplings
+    # counted once per system, as if every temperature had its own)
+    single_basis = (NT * N * 40) / launches / (launch_us * 1e-6) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                    kernel=roofline_kernel(fused), launch_us=round(launch_us, 2), alg_bytes_per_launch=alg_bytes)
+                    kernel=roofline_kernel(fused), launch_us=round(launch_us, 2), alg_bytes_per_launch=alg_bytes,
+                    frac_single_lattice_basis=round(single_basis / HBM_PEAK_GBS, 4))
     pt.close()
     batched = None if args.no_batch else batched_vs_plain(sa)
     served = None if args.no_serve else serving(sa, args.serve_requests, args.serve_rate)

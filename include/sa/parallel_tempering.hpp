@@ -140,6 +140,17 @@ class parallel_tempering {
   }
   // records (pair, shared uniform, mask) of every exchange attempt when set
   void set_swap_log(std::vector<swap_record> *log) { log_ = log; }
+  // file-name pattern of the energy files (% -> T<temperature> / E_min); the
+  // reference fixes it at construction
+  void set_filename(const std::string &f) { fname_ = f; }
+  // flush and close the energy files (parallel_tempering::close_file,
+  // src/parallel_tempering.hpp:250-262)
+  void close_file() {
+    files_.clear();
+    if (file_E_min_) file_E_min_->close();
+    file_E_min_.reset();
+    print_E_ = print_min_ = false;
+  }
 
   // Insert a temperature between neighbours whose exchange rate is below 0.3,
   // cloning the hotter neighbour (src/parallel_tempering.hpp:381-413).

@@ -182,6 +182,18 @@ int64_t sa_pt_steps_done(const sa_pt *pt);
 int sa_pt_result(sa_pt *pt, int64_t capacity, int32_t *pairs, double *us, uint64_t *masks, float *E_last,
                  uint64_t *spins);
 sa_ladder *sa_pt_ladder(sa_pt *pt);
+/* The driver's text output (parallel_tempering::print / print_swap,
+ * src/parallel_tempering.hpp:117-139,164-170,203-262): energy files named by
+ * `pattern` with % replaced by T<temperature> (E_min for the minimum file).
+ * sa_pt_step_stride runs step(stride): stride > 0 measures every system and
+ * appends a line of 64 energies to each file every `stride` sweeps (stride 0 is
+ * sa_pt_step).  sa_pt_print_swap returns the acceptance report's length and copies
+ * it (NUL-terminated, truncated to capacity) when buf is not NULL. */
+int sa_pt_set_output(sa_pt *pt, const char *pattern);
+int sa_pt_step_stride(sa_pt *pt, int nsteps, int stride);
+int sa_pt_print_min(sa_pt *pt);
+int64_t sa_pt_print_swap(sa_pt *pt, char *buf, int64_t capacity);
+int sa_pt_close_output(sa_pt *pt);
 /* Checkpoint (.tempsys) of the run: parallel_tempering::save / load
  * (src/parallel_tempering.hpp:280-377; main_cpu's -s / --reload files).  With the
  * replay RNG the file is the reference's byte for byte and a reference checkpoint

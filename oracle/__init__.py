@@ -273,6 +273,8 @@ def ref():
         R.ref_pt_checkpoint.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, C.c_double, C.c_long, C.c_int, C.c_int,
                                         C.c_int, C.c_char_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p]
+        R.ref_pt_files.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, C.c_double, C.c_long, C.c_int, C.c_int, C.c_int,
+                                   C.c_char_p, C.c_char_p, C.c_long]
         R.ref_anneal_geometric.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                            C.c_double, C.c_long, C.c_char_p, C.c_long]
         R.ref_time_sweeps.argtypes = [C.c_int, C.c_int, C.c_double, C.c_long, C.c_int]
@@ -361,6 +363,16 @@ def ref_pt_checkpoint(kind, L, T, h, seed, sweeps, nsteps1, nsteps2, fname, path
                                  str(path).encode(), _ptr(pairs), _ptr(us), _ptr(masks), _ptr(spins))
     assert rc == 0
     return dict(pairs=pairs[:nsteps2], us=us[:nsteps2], masks=masks[:nsteps2], spins=spins.reshape(nT, N))
+
+
+def ref_pt_files(kind, L, T, h, seed, sweeps, nsteps, stride, pattern) -> str:
+    """The reference PT driver's energy files (written under `pattern`) and its
+    print_swap text (returned)."""
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    buf = C.create_string_buffer(1 << 16)
+    n = ref().ref_pt_files(kind, L, T.size, T, h, seed, sweeps, nsteps, stride, str(pattern).encode(), buf, len(buf))
+    assert n >= 0
+    return buf.value.decode()
 
 
 def ref_anneal_geometric(kind, L, T0, alfa, sweeps, nsteps, stride, h, seed) -> str:

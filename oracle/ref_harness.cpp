@@ -225,6 +225,39 @@ int ref_pt_checkpoint(int kind, int L, int nT, const double *Tin, double h, long
   return 0;
 }
 
+// The reference PT driver's output files: nsteps x step(stride) with energy files
+// named by `pattern` (% -> T<temperature>), then print(PRINT_MIN) and print_swap
+// into swapbuf.  The files are complete when this returns.
+int ref_pt_files(int kind, int L, int nT, const double *Tin, double h, long seed, int sweeps, int nsteps, int stride,
+                 const char *pattern, char *swapbuf, long buflen) {
+  Logger::verbosity = 0;
+  std::vector<cpu_sys *> sys;
+  for (int i = 0; i < nT; ++i) {
+    cpu_sys *q = make_sys(kind, L, Tin[0], h);
+    q->set_seed(seed);
+    q->init_J();
+    q->set_seed(seed);
+    q->init_rand();
+    sys.push_back(q);
+  }
+  std::vector<double> T(Tin, Tin + nT);
+  std::string fn = pattern;
+  std::string swaps;
+  {
+    parallel_tempering<cpu_sys> pt(sys, T, sweeps, fn);
+    for (int k = 0; k < nsteps; ++k) pt.step(stride);
+    pt.print(PRINT_MIN);
+    std::ostringstream os;
+    pt.print_swap(os);
+    swaps = os.str();
+    pt.close_file();
+  }
+  for (cpu_sys *q : sys) delete q;
+  if ((long)swaps.size() + 1 > buflen) return -1;
+  std::memcpy(swapbuf, swaps.c_str(), swaps.size() + 1);
+  return (int)swaps.size();
+}
+
 // annealing<cpu_sys> with geometric cooling, exactly the reference driver:
 // annealing(T0, sweeps, alfa, sys, out); nsteps x step(stride, PRINT_E).  Writes the
 // driver's text output (one line of 64 energies per measurement) into buf.

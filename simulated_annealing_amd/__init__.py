@@ -35,7 +35,8 @@ EXPORTS = [
     "sa_ladder_set_tuning", "sa_ladder_get_tuning",
     "sa_pt_run", "sa_anneal_run",
     "sa_pt_create", "sa_pt_step", "sa_pt_sync", "sa_pt_steps_done", "sa_pt_result", "sa_pt_ladder",
-    "sa_pt_save", "sa_pt_load", "sa_pt_destroy",
+    "sa_pt_save", "sa_pt_load", "sa_pt_destroy", "sa_pt_set_output", "sa_pt_step_stride", "sa_pt_print_min",
+    "sa_pt_print_swap", "sa_pt_close_output",
     "sa_batch_create", "sa_batch_destroy", "sa_batch_add", "sa_batch_add_pt", "sa_batch_run",
     "sa_batch_sweeps_done", "sa_batch_get_spins", "sa_batch_measure", "sa_batch_pt_result", "sa_batch_bytes",
     "sa_batch_ladder", "sa_batch_remove", "sa_batch_stream", "sa_batch_set_tuning", "sa_batch_get_tuning",
@@ -136,6 +137,12 @@ def lib():
         L.sa_pt_result.argtypes = [_vp, C.c_int64, _vp, _vp, _vp, _vp, _vp]
         L.sa_pt_ladder.argtypes = [_vp]
         L.sa_pt_ladder.restype = _vp
+        L.sa_pt_set_output.argtypes = [_vp, C.c_char_p]
+        L.sa_pt_step_stride.argtypes = [_vp, C.c_int, C.c_int]
+        L.sa_pt_print_min.argtypes = [_vp]
+        L.sa_pt_print_swap.argtypes = [_vp, C.c_char_p, C.c_int64]
+        L.sa_pt_print_swap.restype = C.c_int64
+        L.sa_pt_close_output.argtypes = [_vp]
         L.sa_pt_save.argtypes = [_vp, C.c_char_p]
         L.sa_pt_load.argtypes = [_vp, C.c_char_p]
         L.sa_pt_destroy.argtypes = [_vp]
@@ -349,6 +356,26 @@ class PT:
 
     def ladder_handle(self):
         return lib().sa_pt_ladder(self.p)
+
+    def set_output(self, pattern):
+        """File-name pattern of the energy files (% -> T<temperature>, E_min)."""
+        check(lib().sa_pt_set_output(self.p, str(pattern).encode()), "sa_pt_set_output")
+
+    def step_stride(self, n, stride):
+        """n x parallel_tempering::step(stride): energies printed every `stride` sweeps."""
+        check(lib().sa_pt_step_stride(self.p, n, stride), "sa_pt_step_stride")
+
+    def print_min(self):
+        check(lib().sa_pt_print_min(self.p), "sa_pt_print_min")
+
+    def swap_report(self) -> str:
+        n = check(lib().sa_pt_print_swap(self.p, None, 0), "sa_pt_print_swap")
+        buf = C.create_string_buffer(n + 1)
+        lib().sa_pt_print_swap(self.p, buf, n + 1)
+        return buf.value.decode()
+
+    def close_output(self):
+        check(lib().sa_pt_close_output(self.p), "sa_pt_close_output")
 
     def save(self, path):
         """parallel_tempering::save to a file (.tempsys)."""

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -94,6 +95,46 @@ int sa_pt_result(sa_pt *p, int64_t capacity, int32_t *pairs, double *us, uint64_
       if (rc != SA_OK) return rc;
     }
   }
+  return SA_OK;
+}
+
+int sa_pt_set_output(sa_pt *p, const char *pattern) {
+  if (!p || !pattern) return SA_ERR_ARG;
+  p->pt->close_file();
+  p->fname = pattern;
+  p->pt->set_filename(p->fname);
+  return SA_OK;
+}
+
+int sa_pt_step_stride(sa_pt *p, int nsteps, int stride) {
+  if (!p || nsteps < 0 || stride < 0) return SA_ERR_ARG;
+  for (int k = 0; k < nsteps; ++k) p->pt->step(stride);
+  p->done += nsteps;
+  return SA_OK;
+}
+
+int sa_pt_print_min(sa_pt *p) {
+  if (!p) return SA_ERR_ARG;
+  p->pt->print(sa::PRINT_MIN);
+  return SA_OK;
+}
+
+int64_t sa_pt_print_swap(sa_pt *p, char *buf, int64_t capacity) {
+  if (!p) return SA_ERR_ARG;
+  std::ostringstream os;
+  p->pt->print_swap(os);
+  const std::string t = os.str();
+  if (buf && capacity > 0) {
+    const int64_t n = std::min<int64_t>(capacity - 1, (int64_t)t.size());
+    std::memcpy(buf, t.data(), n);
+    buf[n] = '\0';
+  }
+  return (int64_t)t.size();
+}
+
+int sa_pt_close_output(sa_pt *p) {
+  if (!p) return SA_ERR_ARG;
+  p->pt->close_file();
   return SA_OK;
 }
 

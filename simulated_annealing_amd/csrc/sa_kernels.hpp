@@ -335,6 +335,7 @@ struct fused_ctx {
   const sweep_args &p;
   uint64_t *lds;
   int L, PR, r, rn, rs, k, kn, kp, y, row, rowS, yr2, sys0;
+  uint32_t t_rk, t_rkp, t_rkn, t_rnk, t_rsk;  // per-thread LDS byte offsets (see at())
   uint64_t t4[MB], t5[MB], t6[MB];  // FAST: the thresholded levels of each system's table
   bool halo;
   int64_t L2, N, rank0;
@@ -343,15 +344,16 @@ struct fused_ctx {
   uint32_t vpair, vpairS, vnext, vring, vrank, vrank1, vstore;  // per-thread byte offsets
   uint32_t plane_bytes;                          // L*L*8
 
-  __device__ __forceinline__ uint64_t &c0(int buf, int rr, int m, int kk) const {
-    return lds[(((buf * (TH + 2) + rr) * MB + m) * 2 + 0) * PR + kk];
-  }  // buf = plane % 3
-  __device__ __forceinline__ uint64_t &c1(int buf, int rr, int m, int kk) const {
-    return lds[(((buf * (TH + 2) + rr) * MB + m) * 2 + 1) * PR + kk];
+  // LDS word of (plane buffer, system m, colour c) at a thread position.  The
+  // element index (((buf*(TH+2) + row)*MB + m)*2 + c)*PR + col splits into a
+  // wave-uniform slab offset (buf, m, c) and a per-thread offset (row, col) that is
+  // fixed for the walk (t_rk .. t_rsk), so an access costs one add.
+  __device__ __forceinline__ uint32_t slab(int buf, int m, int c) const {
+    return (uint32_t)((buf * (TH + 2) * MB * 2 + m * 2 + c) * PR) * 8u;
   }
-
-  // loads of plane z; no branches (see kOutOfRange): above the top plane the own
-  // pair reads as 0, ring2 is 0 outside the halo rows, g1 is 0 in the halo rows
+  __device__ __forceinline__ uint64_t &at(uint32_t slab_off, uint32_t toff) const {
+    return *reinterpret_cast<uint64_t *>(reinterpret_cast<char *>(lds) + slab_off + toff);
+  }
   __device__ __forceinline__ void load(int z, fused_loads<MB> &q) const {
     const bool top = z == L - 1;
     const uint32_t P = (uint32_t)z * plane_bytes;
@@ -395,9 +397,10 @@ struct fused_ctx {
     const int buf = q % 3;
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
-      v.w[m] = c1(buf, r, m, P ? k : kp);
-      v.e[m] = c1(buf, r, m, P ? kn : k);
-      const uint64_t nn = c1(buf, rn, m, k), ss = c1(buf, rs, m, k);
+      const uint32_t sl = slab(buf, m, 1);
+      v.w[m] = at(sl, P ? t_rk : t_rkp);
+      v.e[m] = at(sl, P ? t_rkn : t_rk);
+      const uint64_t nn = at(sl, t_rnk), ss = at(sl, t_rsk);
       v.n[m] = (r == 0) ? ld.ring2[m] : nn;
       v.s[m] = (r == TH + 1) ? ld.ring2[m] : ss;
     }
@@ -408,10 +411,11 @@ struct fused_ctx {
     const int buf = q % 3;
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
-      v.w[m] = c0(buf, r, m, P ? kp : k);
-      v.e[m] = c0(buf, r, m, P ? k : kn);
-      v.n[m] = c0(buf, rn, m, k);
-      v.s[m] = c0(buf, rs, m, k);
+      const uint32_t sl = slab(buf, m, 0);
+      v.w[m] = at(sl, P ? t_rkp : t_rk);
+      v.e[m] = at(sl, P ? t_rk : t_rkn);
+      v.n[m] = at(sl, t_rnk);
+      v.s[m] = at(sl, t_rsk);
     }
   }
 
@@ -438,7 +442,7 @@ struct fused_ctx {
                               xor3(ju0, si, up0[m]), xor3(j0d, si, down));
       const uint64_t n0 =
           FAST ? update_fast3t(t4[m], t5[m], t6[m], g0, c, si) : update_general(p.tab[sys0 + m], 0, g0, c, si, 7);
-      c0(buf, r, m, k) = n0;
+      at(slab(buf, m, 0), t_rk) = n0;
       up0[m] = n0;
       if (q == 0) first0[m] = n0;
       bs.si[m] = P ? pa[m] : pb[m];
@@ -483,8 +487,8 @@ struct fused_ctx {
     const int buf = q % 3;
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
-      c0(buf, r, m, k) = P ? pb[m] : pa[m];
-      c1(buf, r, m, k) = P ? pa[m] : pb[m];
+      at(slab(buf, m, 0), t_rk) = P ? pb[m] : pa[m];
+      at(slab(buf, m, 1), t_rk) = P ? pa[m] : pb[m];
     }
   }
 
@@ -617,6 +621,14 @@ __global__ void __launch_bounds__(640) k_sweep3d_fused(sweep_args p, uint64_t *_
   c.halo = (c.r == 0) || (c.r == TH + 1);
   c.rn = c.r == 0 ? 0 : c.r - 1;
   c.rs = c.r == TH + 1 ? TH + 1 : c.r + 1;
+  {
+    const uint32_t rowst = (uint32_t)(MB * 2 * PR);  // elements per LDS row (all systems, both colours)
+    c.t_rk = ((uint32_t)c.r * rowst + (uint32_t)c.k) * 8u;
+    c.t_rkp = ((uint32_t)c.r * rowst + (uint32_t)c.kp) * 8u;
+    c.t_rkn = ((uint32_t)c.r * rowst + (uint32_t)c.kn) * 8u;
+    c.t_rnk = ((uint32_t)c.rn * rowst + (uint32_t)c.k) * 8u;
+    c.t_rsk = ((uint32_t)c.rs * rowst + (uint32_t)c.k) * 8u;
+  }
   c.y = (tile * TH - 1 + c.r + L) % L;
   const int ym = c.y == 0 ? L - 1 : c.y - 1, yp = c.y == L - 1 ? 0 : c.y + 1;
   c.row = c.y * L;

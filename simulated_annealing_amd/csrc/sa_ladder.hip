@@ -35,9 +35,7 @@ int bonds_per_site(int kind) { return kind == SA_KIND_3D ? 3 : 2; }
 
 void free_all(sa_ladder *l) {
   if (l->s) (void)hipFree(l->s);
-  if (l->jx) (void)hipFree(l->jx);
-  if (l->jy) (void)hipFree(l->jy);
-  if (l->jz) (void)hipFree(l->jz);
+  if (l->jx) (void)hipFree(l->jx);  // jy, jz live in the same allocation
   if (l->tab) (void)hipFree(l->tab);
   if (l->cnt) (void)hipFree(l->cnt);
   if (l->s_alt) (void)hipFree(l->s_alt);
@@ -124,7 +122,10 @@ bool fused_ok(const sa_ladder *l, int th, int mb) {
 // systems per workgroup when the ladder has enough of them to fill the GPU
 fused_cfg pick_fused(const sa_ladder *l) {
   fused_cfg c{0, 1, 0};
-  const int mbs[2] = {l->sys_per_thread > 0 ? l->sys_per_thread : 2, 1};
+  // general tables (h != 0) need more registers: one system per thread unless asked
+  bool all_fast = true;
+  for (char f : l->fast) all_fast = all_fast && f;
+  const int mbs[2] = {l->sys_per_thread > 0 ? l->sys_per_thread : (all_fast ? 2 : 1), 1};
   const int ths[3] = {16, 8, 4};
   for (int mb : mbs) {
     if (mb != 1 && mb != 2) continue;
@@ -259,19 +260,18 @@ int sa_ladder_create(int kind, int L, int nsys, sa_ladder **out) {
     return true;
   };
   if (bad(hipGetDevice(&l->device), "hipGetDevice") || bad(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking), "hipStreamCreate") ||
-      bad(hipMalloc(&l->s, (size_t)nsys * N * 8), "spins") || bad(hipMalloc(&l->jx, N * 8), "jx") ||
-      bad(hipMalloc(&l->jy, N * 8), "jy") || bad(hipMalloc(&l->jz, N * 8), "jz") ||
+      bad(hipMalloc(&l->s, (size_t)nsys * N * 8), "spins") || bad(hipMalloc(&l->jx, 3 * N * 8), "couplings") ||
       bad(hipMalloc(&l->tab, sizeof(sa::sys_table) * nsys), "tables") ||
       bad(hipMalloc(&l->cnt, sizeof(int) * 128 * nsys), "counters") ||
       bad(hipMalloc(&l->sys_list, sizeof(int) * nsys), "system list") ||
       bad(hipMemsetAsync(l->s, 0, (size_t)nsys * N * 8, l->stream), "memset") ||
-      bad(hipMemsetAsync(l->jx, 0, N * 8, l->stream), "memset") ||
-      bad(hipMemsetAsync(l->jy, 0, N * 8, l->stream), "memset") ||
-      bad(hipMemsetAsync(l->jz, 0, N * 8, l->stream), "memset")) {
+      bad(hipMemsetAsync(l->jx, 0, 3 * N * 8, l->stream), "memset")) {
     free_all(l);
     delete l;
     return rc;
   }
+  l->jy = l->jx + N;  // one allocation for the three coupling arrays
+  l->jz = l->jx + 2 * N;
   l->htab.assign(nsys, sa::sys_table{});
   l->fast.assign(nsys, 0);
   l->cnt_h.assign((size_t)128 * nsys, 0);

@@ -4,6 +4,7 @@ sources).  Run in the container that has the reference sources:
 
     python tests/golden/make_golden.py                 # ref_vectors.json
     python tests/golden/make_golden.py --checkpoints   # ckpt_vectors.json + ckpt_*.tempsys
+    python tests/golden/make_golden.py --pt-files      # pt_files.json
 
 The fixtures are data only: inputs, seeds and the reference's outputs.  Spin arrays
 are stored whole (hex words) for small lattices and as SHA-256 of their
@@ -24,6 +25,7 @@ import oracle as O  # noqa: E402
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.json")
 CKPT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ckpt_vectors.json")
+PTF_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pt_files.json")
 
 
 def hexw(a) -> list[str]:
@@ -158,9 +160,32 @@ def checkpoints():
     return out
 
 
+def pt_files():
+    """Energy files, E_min file and print_swap text of the reference PT driver run
+    with a stride (print(PRINT_E) every `stride` sweeps)."""
+    import glob
+    import tempfile
+
+    out = []
+    for kind, L, nT, sweeps, nsteps, stride, h in [(2, 6, 3, 6, 4, 2, 0.0), (1, 8, 4, 5, 3, 1, 0.25)]:
+        T = [0.5 * 4 ** (k / (nT - 1)) for k in range(nT)]
+        with tempfile.TemporaryDirectory() as d:
+            swaps = O.ref_pt_files(kind, L, np.array(T), h, 555, sweeps, nsteps, stride, os.path.join(d, "pt_%.dat"))
+            files = {os.path.basename(f): open(f).read() for f in sorted(glob.glob(os.path.join(d, "*")))}
+        out.append(dict(kind=kind, L=L, T=T, h=h, seed=555, sweeps=sweeps, nsteps=nsteps, stride=stride,
+                        swap_text=swaps, files=files))
+    return out
+
+
 def main():
     if not O.ref_available():
         O.build_ref()
+    if "--pt-files" in sys.argv:  # only the PT output-file fixtures
+        with open(PTF_OUT, "w") as f:
+            json.dump(dict(about="parallel_tempering output files of the reference (see make_golden.py).",
+                           runs=pt_files()), f, indent=0, separators=(",", ":"))
+        print("wrote", PTF_OUT)
+        return
     if "--checkpoints" in sys.argv:  # only the checkpoint fixtures
         with open(CKPT_OUT, "w") as f:
             json.dump(dict(about="parallel_tempering::save records of the reference (see make_golden.py).",

@@ -307,3 +307,40 @@ def test_measure_systems_subset_matches_full_measure(sa):
             assert (Es[q] == E[j]).all() and (Ms[q] == M[j]).all()
     with pytest.raises(sa.SAError):
         lad.measure_systems([6])
+
+
+def test_ladders_swept_from_concurrent_threads(sa):
+    """Calls on different ladders may come from different threads (the reference's
+    parallel_tempering sweeps systems from OpenMP threads): four ladders swept
+    concurrently end exactly as when swept one after another."""
+    import threading
+
+    cases = [(2, 16, 2, 11), (1, 32, 3, 12), (0, 24, 1, 13), (2, 8, 4, 14)]
+
+    def make(kind, L, nsys, seed):
+        lad = sa.Ladder(kind, L, nsys)
+        lad.set_rng(sa.RNG_COUNTER)
+        lad.seed_like_main(seed)
+        for j in range(nsys):
+            lad.set_temperature(j, 0.6 + 0.4 * j)
+        return lad
+
+    def work(lad, out, q):
+        for _ in range(20):
+            lad.sweep(1)
+            lad.measure()
+        out[q] = [lad.spins(j) for j in range(lad.nsys)]
+
+    seq = {}
+    for q, c in enumerate(cases):
+        work(make(*c), seq, q)
+    par = {}
+    lads = [make(*c) for c in cases]
+    ths = [threading.Thread(target=work, args=(lad, par, q)) for q, lad in enumerate(lads)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    for q in range(len(cases)):
+        for a, b in zip(seq[q], par[q]):
+            assert (a == b).all(), cases[q]
