@@ -500,10 +500,13 @@ int measure_list(sa_ladder *l, const int *list, int n) {
     SA_HIP_TRY(hipMemcpyAsync(l->sys_list, list, sizeof(int) * n, hipMemcpyHostToDevice, l->stream));
     dlist = l->sys_list;
   }
-  // cnt layout [listed system][2][64]; one block row per 4 systems, ~2 waves/SIMD
+  // cnt layout [listed system][2][64]; one block row per 4 systems, ~2 waves/SIMD.
+  // A wave converts its bit-sliced counters (a 64x64 transpose per plane) after 16
+  // site groups or at its end, so each wave gets a full 16 groups where possible:
+  // with few systems more blocks would only add conversions.
   const int yb = (n + 3) / 4;
   const int64_t want = std::max<int64_t>(1, 2048 / yb);
-  const int64_t blocks = std::min<int64_t>((l->N + 255) / 256, want);
+  const int64_t blocks = std::min<int64_t>((l->N + 256 * 16 - 1) / (256 * 16), want);
   dim3 grid((unsigned)blocks, yb);
   if (l->kind == SA_KIND_3D)
     hipLaunchKernelGGL(sa::k_measure<sa::KIND_3D>, grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, l->L,
