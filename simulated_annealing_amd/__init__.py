@@ -19,7 +19,7 @@ KIND_2P, KIND_2D, KIND_3D = 0, 1, 2
 RNG_REPLAY, RNG_COUNTER = 0, 1
 SA_OK = 0
 TUNE_SYS_PER_THREAD, TUNE_SWEEP3D_KERNEL, TUNE_ROWS_PER_TILE, TUNE_XCD_MAP = 1, 2, 3, 4
-TUNE_BATCH_BLOCK, TUNE_BATCH_CHUNK = 101, 102
+TUNE_BATCH_BLOCK, TUNE_BATCH_CHUNK, TUNE_BATCH_FUSED = 101, 102, 103
 
 # every symbol include/sa_hip.h declares (checked by tests/test_capi_symbols.py)
 EXPORTS = [

@@ -39,13 +39,18 @@
 namespace {
 
 struct inst_desc {
-  sa::sweep_args a;  // tab: the instance's tables (per sweep: nsweeps x nsys, else nsys)
-  int kind, nsweeps, t, per_sweep, fast, pad;
+  sa::sweep_args a;  // a.s: first spin array; tab: tables (per sweep: nsweeps x nsys, else nsys)
+  uint64_t *s2;      // second spin array (instances that may take the fused 3D sweep)
+  int kind, nsweeps, t, per_sweep, fast;
+  int fused;  // swept by the fused kernel in this schedule (reads the current array, writes the other)
+  int cur;    // 1: s2 holds the current spins
+  int pad;
 };
 
 template <bool FAST>
 __device__ __forceinline__ void batch_item(const inst_desc &in, int color, int k, int sys) {
   sa::sweep_args a = in.a;
+  if (in.cur) a.s = in.s2;  // half-sweeps update the current array in place
   if (in.per_sweep) a.tab += (int64_t)in.t * a.nsys;
   a.sweep = (uint64_t)in.t;
   if (in.kind == SA_KIND_3D) sa::half3d_column<true, FAST, 1>(a, color, k, sys);
@@ -73,7 +78,36 @@ __global__ void __launch_bounds__(256) k_batch_half(const inst_desc *__restrict_
 __global__ void k_batch_advance(inst_desc *d, const int *__restrict__ ctrl) {
   const int n = ctrl[1];
   for (int i = threadIdx.x; i < n; i += blockDim.x)
-    if (d[i].t < d[i].nsweeps) d[i].t += 1;
+    if (d[i].t < d[i].nsweeps) {
+      d[i].t += 1;
+      if (d[i].fused) d[i].cur ^= 1;  // the fused sweep wrote the other array
+    }
+}
+
+// Fused 3D sweeps of one group of instances (same L, band height TH and systems per
+// workgroup MB, so one block size and LDS size): group g's blocks are entries
+// ctrl[2+2g] .. +ctrl[3+2g] of the fused block map, each (descriptor, band index
+// sb * (L/TH) + tile).  Each block runs the single-ladder fused walk on its band.
+constexpr int kMaxFusedGroups = 24;
+template <int MB, int TH>
+__global__ void __launch_bounds__(640) k_batch_fused(const inst_desc *__restrict__ d, const int2 *__restrict__ fmap,
+                                                     const int *__restrict__ ctrl, int g) {
+  extern __shared__ uint64_t lds[];
+  const int off = ctrl[2 + 2 * g], n = ctrl[3 + 2 * g];
+  for (int blk = blockIdx.x; blk < n; blk += gridDim.x) {
+    const int2 bm = fmap[off + blk];
+    const inst_desc &in = d[bm.x];
+    if (in.t >= in.nsweeps) continue;  // the whole block skips together
+    sa::sweep_args a = in.a;
+    if (in.per_sweep) a.tab += (int64_t)in.t * a.nsys;
+    a.sweep = (uint64_t)in.t;
+    a.s = in.cur ? in.s2 : in.a.s;
+    uint64_t *out = in.cur ? in.a.s : in.s2;
+    const int ntile = a.L / TH;
+    const int sb = bm.y / ntile, tile = bm.y - sb * ntile;
+    sa::fused_band<true, true, MB, TH>(a, out, sb, tile, lds);
+    __syncthreads();  // the next band reuses the LDS
+  }
 }
 
 // ---- measurement of listed systems of many instances
@@ -234,6 +268,7 @@ struct pt_state {
 
 struct batch_inst {
   sa_ladder *lad = nullptr;
+  uint64_t *s_first = nullptr;  // the ladder's spin array at attach (s_alt is the other)
   sa::sys_table *tabs = nullptr;
   int nsweeps = 0, per_sweep = 0, t = 0;
   bool fast = true, live = false;
@@ -242,8 +277,19 @@ struct batch_inst {
   pt_state pt;
 };
 
-const int kBlocks[3] = {64, 128, 256};
-const int kChunks[2] = {8, 32};
+// The fused 3D sweep inside the batch (k_batch_fused on forked graph branches) is
+// switched off: one run of its test stopped making progress and no cause has been
+// found in this code.  Off, 3D instances take the half-sweep kernels, no second
+// spin array is allocated and SA_TUNE_BATCH_FUSED = 2 is refused.
+constexpr bool kFusedBatch = false;
+
+// online-tuning candidates: (fused 3D sweep 2 on / 1 off, threads per half-sweep
+// block, sweeps per graph); the first chunks try them in this order
+struct tune_cand {
+  int fused, block, chunk;
+};
+const tune_cand kCands[4] = {{1, 128, 32}, {1, 256, 32}, {1, 64, 32}, {1, 128, 8}};
+const int kNumCands = 4;
 
 }  // namespace
 
@@ -257,18 +303,30 @@ struct sa_batch {
   dev_array<int> d_ctrl;
   int ndesc = 0, nblocks = 0, bmap_bs = 0;
   std::vector<int> desc_id;  // instance id of each descriptor
+  std::vector<char> desc_fused;  // descriptor swept by the fused kernel
+  // fused 3D groups of the current schedule: (L, TH, MB) and block count each
+  struct fgroup {
+    int L, TH, MB, count;
+  };
+  std::vector<fgroup> groups;
+  dev_array<int2> d_fmap;
+  int sched_fused = -1;  // fused mode the schedule was built with
   bool dirty = true;
-  std::map<int, hipGraphExec_t> graphs;  // chunk length -> graph over the current arrays and block size
+  std::map<int, hipGraphExec_t> graphs;  // chunk length -> graph over the current arrays, block size, groups
   int graph_bs = 0;
+  std::vector<fgroup> graph_groups;
+  std::vector<hipStream_t> side;  // side streams of the fused groups (fork/join inside the graph)
+  std::vector<hipEvent_t> side_done;
+  hipEvent_t fork = nullptr;
   // exchange staging
   pinned h_stage;
   dev_array<meas_item> d_meas;
   dev_array<swap_item> d_swap;
   dev_array<int> d_cnt;
   pinned h_cnt;
-  // tuning: 0 = automatic
-  int block = 0, chunk = 0;
-  int tuned_block = 0, tuned_chunk = 0;
+  // tuning: 0 = automatic; fused: 0 automatic, 1 off, 2 on
+  int block = 0, chunk = 0, fused = 0;
+  int tuned_block = 0, tuned_chunk = 0, tuned_fused = 0;
   std::vector<std::pair<int, float>> trials;  // (candidate, ms per work item)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int64_t bytes_live = 0;
@@ -294,7 +352,23 @@ int grid_for(const sa_batch *b, int bs) {
 }
 
 // descriptors and block map of the unfinished instances, largest remaining work first
-int rebuild(sa_batch *b, int bs) {
+// fused 3D configuration of an instance: the tallest band (TH rows) that fits the
+// kernel's launch bound and LDS, two systems per workgroup when nsys is even
+bool fused_cfg(const sa_ladder *l, int &TH, int &MB) {
+  if (l->kind != SA_KIND_3D || (int64_t)l->N * 8 >= ((int64_t)1 << 31)) return false;
+  MB = (l->nsys % 2 == 0) ? 2 : 1;
+  for (int th : {16, 8, 4}) {
+    if (l->L % th || th > l->L || (th + 2) * (l->L / 2) > 640) continue;
+    if ((size_t)3 * (th + 2) * MB * 2 * (l->L / 2) * 8 > 64 * 1024) continue;
+    TH = th;
+    return true;
+  }
+  return false;
+}
+
+// descriptors, the half-sweep block map and the fused groups' block maps of the
+// unfinished instances, largest remaining work first
+int rebuild(sa_batch *b, int bs, bool fused) {
   std::vector<int> ids;
   for (int i = 0; i < (int)b->inst.size(); ++i)
     if (b->inst[i].live && b->inst[i].t < b->inst[i].nsweeps) ids.push_back(i);
@@ -302,30 +376,71 @@ int rebuild(sa_batch *b, int bs) {
     const auto &a = b->inst[x], &c = b->inst[y];
     return a.work * (int64_t)(a.nsweeps - a.t) > c.work * (int64_t)(c.nsweeps - c.t);
   });
+  // which instances take the fused sweep, and their groups
+  std::vector<sa_batch::fgroup> groups;
+  std::vector<std::vector<int2>> gmaps;
+  std::vector<int> gidx(ids.size(), -1);
   int64_t nblk = 0;
-  for (int id : ids) nblk += (b->inst[id].work + bs - 1) / bs;
-  if (nblk > INT32_MAX / 2) return sa_fail(SA_ERR_ARG, "batch too large");
+  for (size_t q = 0; q < ids.size(); ++q) {
+    const batch_inst &in = b->inst[ids[q]];
+    int TH = 0, MB = 0;
+    if (fused && in.fast && in.lad->s_alt && fused_cfg(in.lad, TH, MB)) {
+      int g = -1;
+      for (size_t j = 0; j < groups.size(); ++j)
+        if (groups[j].L == in.lad->L && groups[j].TH == TH && groups[j].MB == MB) g = (int)j;
+      if (g < 0 && (int)groups.size() < kMaxFusedGroups) {
+        g = (int)groups.size();
+        groups.push_back({in.lad->L, TH, MB, 0});
+        gmaps.emplace_back();
+      }
+      if (g >= 0) {
+        gidx[q] = g;
+        const int nb = (in.lad->L / TH) * (in.lad->nsys / MB);
+        for (int k = 0; k < nb; ++k) gmaps[g].push_back(make_int2((int)q, k));
+        groups[g].count += nb;
+        continue;
+      }
+    }
+    nblk += (in.work + bs - 1) / bs;
+  }
+  int64_t nf = 0;
+  for (auto &m : gmaps) nf += (int64_t)m.size();
+  if (nblk > INT32_MAX / 2 || nf > INT32_MAX / 2) return sa_fail(SA_ERR_ARG, "batch too large");
   // earlier launches may still read the old arrays and the staging buffer
   SA_HIP_TRY(hipStreamSynchronize(b->stream));
-  bool m1 = false, m2 = false, m3 = false;
+  bool m1 = false, m2 = false, m3 = false, m4 = false;
   int rc = b->d_desc.ensure(std::max<size_t>(ids.size(), 1), m1);
   if (!rc) rc = b->d_bmap.ensure(std::max<int64_t>(nblk, 1), m2);
-  if (!rc) rc = b->d_ctrl.ensure(2, m3);
+  if (!rc) rc = b->d_ctrl.ensure(2 + 2 * kMaxFusedGroups, m3);
+  if (!rc) rc = b->d_fmap.ensure(std::max<int64_t>(nf, 1), m4);
   if (rc) return rc;
-  if (m1 || m2 || m3 || b->graph_bs != bs) drop_graphs(b);
+  bool same_groups = groups.size() == b->graph_groups.size();
+  for (size_t j = 0; same_groups && j < groups.size(); ++j)
+    same_groups = groups[j].L == b->graph_groups[j].L && groups[j].TH == b->graph_groups[j].TH &&
+                  groups[j].MB == b->graph_groups[j].MB && groups[j].count <= b->graph_groups[j].count;
+  if (m1 || m2 || m3 || m4 || b->graph_bs != bs || !same_groups) {
+    drop_graphs(b);
+    b->graph_groups = groups;  // block counts at capture size the fused grids
+  }
   b->graph_bs = bs;
-  const size_t bytes = sizeof(int) * 2 + sizeof(inst_desc) * ids.size() + sizeof(int2) * nblk + 64;
+  const int nctrl = 2 + 2 * kMaxFusedGroups;
+  const size_t bytes = sizeof(int) * nctrl + sizeof(inst_desc) * ids.size() + sizeof(int2) * (nblk + nf) + 64;
   if (!b->h_stage.ensure(bytes)) return sa_fail(SA_ERR_NOMEM, "pinned staging");
   int *ctrl = (int *)b->h_stage.p;
-  inst_desc *desc = (inst_desc *)((char *)b->h_stage.p + 16);
+  inst_desc *desc = (inst_desc *)((char *)b->h_stage.p + ((sizeof(int) * nctrl + 15) & ~(size_t)15));
   int2 *bmap = (int2 *)(desc + ids.size());
+  int2 *fmap = bmap + nblk;
   int64_t q2 = 0;
+  b->desc_fused.assign(ids.size(), 0);
   for (size_t q = 0; q < ids.size(); ++q) {
     const batch_inst &in = b->inst[ids[q]];
     const sa_ladder *l = in.lad;
     inst_desc d;
     std::memset(&d, 0, sizeof d);
-    d.a.s = l->s;
+    // a.s is the ladder's first array; the current one is l->s
+    d.a.s = in.s_first;
+    d.s2 = (l->s == in.s_first) ? l->s_alt : l->s;
+    d.cur = (l->s == in.s_first) ? 0 : 1;
     d.a.jx = l->jx;
     d.a.jy = l->jy;
     d.a.jz = l->jz;
@@ -341,32 +456,92 @@ int rebuild(sa_batch *b, int bs) {
     d.t = in.t;
     d.per_sweep = in.per_sweep;
     d.fast = in.fast ? 1 : 0;
+    d.fused = gidx[q] >= 0 ? 1 : 0;
     desc[q] = d;
-    for (int64_t off = 0; off < in.work; off += bs) bmap[q2++] = make_int2((int)q, (int)off);
+    b->desc_fused[q] = (char)d.fused;
+    if (!d.fused)
+      for (int64_t off = 0; off < in.work; off += bs) bmap[q2++] = make_int2((int)q, (int)off);
   }
+  std::memset(ctrl, 0, sizeof(int) * nctrl);
   ctrl[0] = (int)nblk;
   ctrl[1] = (int)ids.size();
-  SA_HIP_TRY(hipMemcpyAsync(b->d_ctrl.p, ctrl, sizeof(int) * 2, hipMemcpyHostToDevice, b->stream));
+  int64_t fo = 0;
+  for (size_t g = 0; g < gmaps.size(); ++g) {
+    ctrl[2 + 2 * g] = (int)fo;
+    ctrl[3 + 2 * g] = (int)gmaps[g].size();
+    std::memcpy(fmap + fo, gmaps[g].data(), sizeof(int2) * gmaps[g].size());
+    fo += (int64_t)gmaps[g].size();
+  }
+  SA_HIP_TRY(hipMemcpyAsync(b->d_ctrl.p, ctrl, sizeof(int) * nctrl, hipMemcpyHostToDevice, b->stream));
   if (!ids.empty()) {
     SA_HIP_TRY(hipMemcpyAsync(b->d_desc.p, desc, sizeof(inst_desc) * ids.size(), hipMemcpyHostToDevice, b->stream));
-    SA_HIP_TRY(hipMemcpyAsync(b->d_bmap.p, bmap, sizeof(int2) * nblk, hipMemcpyHostToDevice, b->stream));
+    if (nblk) SA_HIP_TRY(hipMemcpyAsync(b->d_bmap.p, bmap, sizeof(int2) * nblk, hipMemcpyHostToDevice, b->stream));
+    if (nf) SA_HIP_TRY(hipMemcpyAsync(b->d_fmap.p, fmap, sizeof(int2) * nf, hipMemcpyHostToDevice, b->stream));
   }
   b->ndesc = (int)ids.size();
   b->desc_id = ids;
   b->nblocks = (int)nblk;
   b->bmap_bs = bs;
+  b->groups = groups;
+  b->sched_fused = fused ? 1 : 0;
   b->dirty = false;
   return SA_OK;
 }
 
-void enqueue_sweeps(sa_batch *b, int k) {
+void launch_fused_group(sa_batch *b, int g, hipStream_t st) {
+  const sa_batch::fgroup &G = b->graph_groups[g];
+  const dim3 grid((unsigned)std::max(1, G.count)), block((unsigned)((G.TH + 2) * (G.L / 2)));
+  const size_t lds = (size_t)3 * (G.TH + 2) * G.MB * 2 * (G.L / 2) * 8;
+#define SA_FUSED_CASE(MBV, THV)                                                                               \
+  if (G.MB == MBV && G.TH == THV) {                                                                         \
+    hipLaunchKernelGGL((k_batch_fused<MBV, THV>), grid, block, lds, st, b->d_desc.p, b->d_fmap.p,            \
+                       b->d_ctrl.p, g);                                                                     \
+    return;                                                                                                 \
+  }
+  SA_FUSED_CASE(1, 4)
+  SA_FUSED_CASE(1, 8)
+  SA_FUSED_CASE(1, 16)
+  SA_FUSED_CASE(2, 4)
+  SA_FUSED_CASE(2, 8)
+  SA_FUSED_CASE(2, 16)
+#undef SA_FUSED_CASE
+}
+
+// One sweep = the two half-sweep launches and one launch per fused group, then the
+// advance.  The launches touch disjoint instances, so the fused groups go on side
+// streams forked from and joined back to the batch stream (parallel branches of
+// the captured graph) and overlap each other and the half-sweeps.
+// side streams and events for ng fused groups (created outside any capture)
+int ensure_side(sa_batch *b, int ng) {
+  for (int g = (int)b->side.size(); g < ng; ++g) {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    SA_HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    SA_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    b->side.push_back(st);
+    b->side_done.push_back(ev);
+  }
+  if (!b->fork) SA_HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  return SA_OK;
+}
+
+int enqueue_sweeps(sa_batch *b, int k) {
   const int grid = grid_for(b, b->bmap_bs);
+  const int ng = (int)b->graph_groups.size();
   for (int s = 0; s < k; ++s) {
+    if (ng) SA_HIP_TRY(hipEventRecord(b->fork, b->stream));
+    for (int g = 0; g < ng; ++g) {
+      SA_HIP_TRY(hipStreamWaitEvent(b->side[g], b->fork, 0));
+      launch_fused_group(b, g, b->side[g]);
+      SA_HIP_TRY(hipEventRecord(b->side_done[g], b->side[g]));
+    }
     for (int color = 0; color < 2; ++color)
       hipLaunchKernelGGL(k_batch_half, dim3(grid), dim3(b->bmap_bs), 0, b->stream, b->d_desc.p, b->d_bmap.p,
                          b->d_ctrl.p, color);
+    for (int g = 0; g < ng; ++g) SA_HIP_TRY(hipStreamWaitEvent(b->stream, b->side_done[g], 0));
     hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(256), 0, b->stream, b->d_desc.p, b->d_ctrl.p);
   }
+  return SA_OK;
 }
 
 // the graph of k sweeps over the current device arrays (captured on first use)
@@ -377,9 +552,16 @@ int graph_for(sa_batch *b, int k, hipGraphExec_t &out) {
     return SA_OK;
   }
   hipGraph_t g = nullptr;
+  const int src = ensure_side(b, (int)b->graph_groups.size());
+  if (src) return src;
   SA_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  enqueue_sweeps(b, k);
-  SA_HIP_TRY(hipStreamEndCapture(b->stream, &g));
+  const int erc = enqueue_sweeps(b, k);
+  const hipError_t ee = hipStreamEndCapture(b->stream, &g);
+  if (erc) {
+    if (g) (void)hipGraphDestroy(g);
+    return erc;
+  }
+  SA_HIP_TRY(ee);
   hipGraphExec_t e = nullptr;
   const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -511,12 +693,14 @@ int pt_exchange(sa_batch *b, const std::vector<int> &due) {
 // host mirror of the device sweep indices after k more sweeps; collects the
 // parallel-tempering instances whose exchange is due
 void advance_host(sa_batch *b, int k, std::vector<int> &due) {
-  for (int id : b->desc_id) {
+  for (size_t q = 0; q < b->desc_id.size(); ++q) {
+    const int id = b->desc_id[q];
     batch_inst &in = b->inst[id];
     if (!in.live) continue;
     const int before = in.t;
     in.t = std::min(in.nsweeps, in.t + k);
     in.lad->t = (uint64_t)in.t;
+    if (b->desc_fused[q] && ((in.t - before) & 1)) std::swap(in.lad->s, in.lad->s_alt);  // current array
     if (in.t >= in.nsweeps && before < in.nsweeps) b->dirty = true;
     if (in.pt.every && in.t > before && in.t % in.pt.every == 0) due.push_back(id);
   }
@@ -549,6 +733,12 @@ int prepare_instance(int kind, int L, int nsys, uint64_t seed, int nsweeps, cons
     sa_detail::make_table(kind, T[per_sweep && nsweeps == 0 ? q % nsys : q], h, tabs[q], f, nullptr);
     fast = fast && f;
   }
+  // a second spin array for the fused 3D sweep (read one, write the other)
+  int th_ = 0, mb_ = 0;
+  if (!rc && kFusedBatch && fused_cfg(l, th_, mb_) && !l->s_alt && hipMalloc(&l->s_alt, (size_t)nsys * l->N * 8) != hipSuccess) {
+    l->s_alt = nullptr;
+    rc = sa_fail(SA_ERR_NOMEM, "second spin array");
+  }
   in = batch_inst();
   in.lad = l;
   in.nsweeps = nsweeps;
@@ -578,6 +768,7 @@ int prepare_instance(int kind, int L, int nsys, uint64_t seed, int nsweeps, cons
 
 void attach_instance(sa_batch *b, batch_inst &&in, int *id) {
   in.live = true;
+  in.s_first = in.lad->s;
   b->bytes_live += in.bytes;
   b->inst.push_back(std::move(in));
   *id = (int)b->inst.size() - 1;
@@ -646,29 +837,33 @@ void batch_discard(prepared_instance *p) {
 
 int64_t batch_instance_bytes(int kind, int L, int nsys, int nsweeps, int per_sweep) {
   const int ntab = per_sweep ? std::max(1, nsweeps) * nsys : nsys;
-  return ladder_bytes(kind, L, nsys) + (int64_t)sizeof(sa::sys_table) * ntab;
+  const int64_t N = (int64_t)L * L * (kind == SA_KIND_3D ? L : 1);
+  const bool second = kFusedBatch && kind == SA_KIND_3D && L % 4 == 0;  // fused-sweep ping-pong array
+  return ladder_bytes(kind, L, nsys) + (int64_t)sizeof(sa::sys_table) * ntab + (second ? (int64_t)nsys * N * 8 : 0);
 }
 
 int batch_launch_chunk(sa_batch *b, int max_sweeps, int *k_out) {
   *k_out = 0;
-  const int ncand = 3 * 2;
-  // block size and chunk length: fixed by the caller, tuned, or the next trial
+  // fixed by the caller, tuned, or the next trial
   int trial = -1;
-  if ((!b->block || !b->chunk) && !(b->tuned_block && b->tuned_chunk)) {
+  if ((!b->block || !b->chunk || !b->fused) && !(b->tuned_block && b->tuned_chunk && b->tuned_fused)) {
     trial = (int)b->trials.size();
-    if (trial >= ncand) {
+    if (trial >= kNumCands) {
       int best = 0;
-      for (int q = 1; q < ncand; ++q)
+      for (int q = 1; q < kNumCands; ++q)
         if (b->trials[q].second < b->trials[best].second) best = q;
-      b->tuned_block = kBlocks[best / 2];
-      b->tuned_chunk = kChunks[best % 2];
+      b->tuned_fused = kCands[best].fused;
+      b->tuned_block = kCands[best].block;
+      b->tuned_chunk = kCands[best].chunk;
       trial = -1;
     }
   }
-  const int bs = b->block ? b->block : (trial >= 0 ? kBlocks[trial / 2] : b->tuned_block);
-  const int kfull = b->chunk ? b->chunk : (trial >= 0 ? kChunks[trial % 2] : b->tuned_chunk);
-  if (b->dirty || b->bmap_bs != bs) {
-    const int rc = rebuild(b, bs);
+  const int bs = b->block ? b->block : (trial >= 0 ? kCands[trial].block : b->tuned_block);
+  const int kfull = b->chunk ? b->chunk : (trial >= 0 ? kCands[trial].chunk : b->tuned_chunk);
+  const int fmode = b->fused ? b->fused : (trial >= 0 ? kCands[trial].fused : b->tuned_fused);
+  const bool fz = fmode == 2;
+  if (b->dirty || b->bmap_bs != bs || b->sched_fused != (fz ? 1 : 0)) {
+    const int rc = rebuild(b, bs, fz);
     if (rc) return rc;
   }
   if (b->ndesc == 0 || max_sweeps <= 0) return SA_OK;
@@ -749,6 +944,9 @@ int sa_batch_destroy(sa_batch *b) {
     }
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  for (auto e : b->side_done) (void)hipEventDestroy(e);
+  for (auto st : b->side) (void)hipStreamDestroy(st);
   hipStream_t s = b->stream;
   delete b;  // device arrays and staging buffers free themselves
   (void)hipStreamDestroy(s);
@@ -866,6 +1064,10 @@ int sa_batch_set_tuning(sa_batch *b, int key, int value) {
   } else if (key == SA_TUNE_BATCH_CHUNK) {
     if (value < 0 || value > 4096) return sa_fail(SA_ERR_ARG, "batch chunk must be 0 (auto) .. 4096");
     b->chunk = value;
+  } else if (key == SA_TUNE_BATCH_FUSED) {
+    if (value < 0 || value > 2) return sa_fail(SA_ERR_ARG, "batch fused must be 0 (auto), 1 (off) or 2 (on)");
+    if (value == 2 && !kFusedBatch) return sa_fail(SA_ERR_ARG, "the fused batch sweep is switched off in this build");
+    b->fused = value;
   } else {
     return sa_fail(SA_ERR_ARG, "unknown batch tuning key %d", key);
   }
@@ -877,6 +1079,7 @@ int sa_batch_get_tuning(sa_batch *b, int key, int *value) {
   if (!b || !value) return sa_fail(SA_ERR_ARG, "null argument");
   if (key == SA_TUNE_BATCH_BLOCK) *value = b->block ? b->block : b->tuned_block;
   else if (key == SA_TUNE_BATCH_CHUNK) *value = b->chunk ? b->chunk : b->tuned_chunk;
+  else if (key == SA_TUNE_BATCH_FUSED) *value = b->fused ? b->fused : b->tuned_fused;
   else return sa_fail(SA_ERR_ARG, "unknown batch tuning key %d", key);
   return SA_OK;
 }

@@ -591,26 +591,13 @@ struct fused_ctx {
   }
 };
 
+// One workgroup's band `tile` of system block `sb` (systems sb*MB .. sb*MB+MB-1):
+// the whole z-walk.  blockDim.x must be (TH+2)*L/2 and the dynamic LDS
+// 3*(TH+2)*MB*2*(L/2)*8 bytes.
 template <bool COUNTER, bool FAST, int MB, int TH>
-__global__ void __launch_bounds__(640) k_sweep3d_fused(sweep_args p, uint64_t *__restrict__ s_out, int xcd_map) {
-  extern __shared__ uint64_t lds[];
+__device__ __forceinline__ void fused_band(const sweep_args &p, uint64_t *__restrict__ s_out, int sb, int tile,
+                                           uint64_t *lds) {
   const int L = p.L, PR = L >> 1;
-  const int ntile = L / TH;
-  // workgroup -> (system block, band).  Workgroups are dispatched round-robin over
-  // the 8 XCDs (L2 is per XCD).  xcd_map = 0: b = sb*ntile + band, so with ntile a
-  // multiple of 8 every system block of a band runs on one XCD and shares its
-  // couplings in L2.  xcd_map = 1: XCD b%8 gets ntile/8 adjacent bands of every
-  // system block, so the halo rows between those bands are L2 hits too.
-  const int b = blockIdx.x;
-  int sb, tile;
-  if (xcd_map) {
-    const int per = ntile >> 3, xcd = b & 7, q = b >> 3;
-    tile = xcd * per + q % per;
-    sb = q / per;
-  } else {
-    sb = b / ntile;
-    tile = b - sb * ntile;
-  }
   fused_ctx<COUNTER, FAST, MB, TH> c{p, lds};
   c.L = L;
   c.PR = PR;
@@ -672,6 +659,28 @@ __global__ void __launch_bounds__(640) k_sweep3d_fused(sweep_args p, uint64_t *_
   } else {
     c.template walk<-1>();
   }
+}
+
+template <bool COUNTER, bool FAST, int MB, int TH>
+__global__ void __launch_bounds__(640) k_sweep3d_fused(sweep_args p, uint64_t *__restrict__ s_out, int xcd_map) {
+  extern __shared__ uint64_t lds[];
+  const int ntile = p.L / TH;
+  // workgroup -> (system block, band).  Workgroups are dispatched round-robin over
+  // the 8 XCDs (L2 is per XCD).  xcd_map = 0: b = sb*ntile + band, so with ntile a
+  // multiple of 8 every system block of a band runs on one XCD and shares its
+  // couplings in L2.  xcd_map = 1: XCD b%8 gets ntile/8 adjacent bands of every
+  // system block, so the halo rows between those bands are L2 hits too.
+  const int b = blockIdx.x;
+  int sb, tile;
+  if (xcd_map) {
+    const int per = ntile >> 3, xcd = b & 7, q = b >> 3;
+    tile = xcd * per + q % per;
+    sb = q / per;
+  } else {
+    sb = b / ntile;
+    tile = b - sb * ntile;
+  }
+  fused_band<COUNTER, FAST, MB, TH>(p, s_out, sb, tile, lds);
 }
 
 // ---------------------------------------------------------------- 2D half-sweep

@@ -154,6 +154,9 @@ def test_batch_rejects_bad_instances(sa):
         b.add(2, 7, 1, 3, T=np.ones(1))  # odd L is not supported on the device
     with pytest.raises(sa.SAError):
         b.tune(sa.TUNE_BATCH_BLOCK, 96)
+    with pytest.raises(sa.SAError):
+        b.tune(sa.TUNE_BATCH_FUSED, 2)  # the fused batch sweep is switched off
+    b.tune(sa.TUNE_BATCH_FUSED, 1)
     b.close()
 
 
@@ -223,5 +226,80 @@ def test_batch_pt_matches_pt_driver(sa):
         last = int(got["pairs"][-1])
         keep = [j for j in range(nT) if j not in (last, last + 1)]
         assert ((E[keep] + h * M[keep]).astype(np.float32) == r["E"][keep]).all()
+        ref.close()
+    b.close()
+
+
+# 3D instances that take the fused sweep (L % 4 == 0, h = 0) beside ones that cannot
+FUSED_MIX = [
+    (2, 4, 1, True, 9, 0.0),
+    (2, 8, 2, False, 13, 0.0),
+    (2, 12, 1, True, 17, 0.0),
+    (2, 16, 3, False, 11, 0.0),
+    (2, 24, 1, True, 21, 0.0),
+    (2, 32, 2, True, 7, 0.0),
+    (2, 48, 1, True, 6, 0.0),
+    (2, 6, 1, True, 10, 0.0),
+    (2, 16, 1, True, 12, 0.3),
+    (1, 32, 1, True, 15, 0.0),
+]
+
+
+FUSED_OFF = pytest.mark.skip(reason="the fused batch sweep is switched off: one run of its test stopped "
+                              "making progress and no cause was found; it stays off the GPU until one is")
+
+
+@FUSED_OFF
+@pytest.mark.parametrize("fused,chunk", [(2, 1), (2, 5), (2, 32), (1, 8)])
+def test_batch_fused_sweep_matches_lone(sa, fused, chunk):
+    say("batch fused", fused, "chunk", chunk)
+    b = sa.Batch()
+    b.tune(sa.TUNE_BATCH_FUSED, fused)
+    b.tune(sa.TUNE_BATCH_CHUNK, chunk)
+    b.tune(sa.TUNE_BATCH_BLOCK, 128)
+    insts = build_mix(sa, b, FUSED_MIX, 900 + chunk)
+    assert b.run() == 0
+    check_mix(sa, b, insts)
+    b.close()
+
+
+@FUSED_OFF
+def test_batch_fused_mode_switches_mid_run(sa):
+    """Odd and even numbers of fused sweeps between switches: the current spin array
+    is tracked across fused (ping-pong) and half-sweep (in place) chunks."""
+    say("batch fused switching")
+    b = sa.Batch()
+    b.tune(sa.TUNE_BATCH_BLOCK, 64)
+    insts = build_mix(sa, b, FUSED_MIX, 321)
+    for fused, chunk, n in [(2, 3, 3), (1, 2, 4), (2, 1, 1), (2, 4, 5), (1, 3, 3)]:
+        b.tune(sa.TUNE_BATCH_FUSED, fused)
+        b.tune(sa.TUNE_BATCH_CHUNK, chunk)
+        b.run(n)
+    b.tune(sa.TUNE_BATCH_FUSED, 2)
+    assert b.run() == 0
+    check_mix(sa, b, insts)
+    b.close()
+
+
+@FUSED_OFF
+def test_batch_fused_pt_matches_pt_driver(sa):
+    """Parallel tempering through the fused sweep (even nT: two systems per workgroup)."""
+    b = sa.Batch()
+    b.tune(sa.TUNE_BATCH_FUSED, 2)
+    cases = [(2, 16, 6, 5, 4), (2, 8, 3, 4, 5), (2, 24, 4, 10, 2)]
+    ids = []
+    for q, (kind, L, nT, every, nsteps) in enumerate(cases):
+        T = 0.5 * 4.0 ** (np.arange(nT) / (nT - 1))
+        ids.append((b.add_pt(kind, L, T, 60 + q, every, nsteps), kind, L, nT, every, nsteps, T, 60 + q))
+    assert b.run() == 0
+    for (i, kind, L, nT, every, nsteps, T, seed) in ids:
+        say("fused PT", L, nT)
+        ref = sa.PT(kind, L, T, 0.0, seed, every, sa.RNG_COUNTER)
+        ref.step(nsteps)
+        r = ref.result()
+        got = b.pt_result(i)
+        assert (got["pairs"] == r["pairs"]).all() and (got["masks"] == r["masks"]).all()
+        for j in range(nT):
+            assert (b.spins(i, j) == r["spins"][j]).all(), j
         ref.close()
     b.close()
