@@ -354,6 +354,9 @@ struct fused_ctx {
   __device__ __forceinline__ uint64_t &at(uint32_t slab_off, uint32_t toff) const {
     return *reinterpret_cast<uint64_t *>(reinterpret_cast<char *>(lds) + slab_off + toff);
   }
+  // WROW: a row is one whole wave (L/2 == 64); jx of pair k+1 then comes from the
+  // next lane at use time instead of a load
+  template <bool WROW>
   __device__ __forceinline__ void load(int z, fused_loads<MB> &q) const {
     const bool top = z == L - 1;
     const uint32_t P = (uint32_t)z * plane_bytes;
@@ -365,7 +368,7 @@ struct fused_ctx {
       q.ring2[m] = bld(sin[m], vring, P);
     }
     bld2(bjx, vpair, P, q.jxa, q.jxb);
-    q.jxn = bld(bjx, vnext, P);
+    if (!WROW) q.jxn = bld(bjx, vnext, P);
     bld2(bjy, vpair, P, q.jya, q.jyb);
     bld2(bjy, vpairS, P, q.jsa, q.jsb);
     bld2(bjz, vpair, Pd, q.jda, q.jdb);
@@ -430,7 +433,9 @@ struct fused_ctx {
     const int P = PT >= 0 ? PT : par;
     const int buf = q % 3;
     const bool last = q == L - 1;
-    const uint64_t j0w = P ? ld.jxb : ld.jxa, j0e = P ? ld.jxn : ld.jxb;
+    // jx of 2k+2: the next pair's jxa (lane k+1 of the same row) when a row is a wave
+    const uint64_t jxn = PT >= 0 ? __shfl(ld.jxa, (int)((threadIdx.x + 1) & 63)) : ld.jxn;
+    const uint64_t j0w = P ? ld.jxb : ld.jxa, j0e = P ? jxn : ld.jxb;
     const uint64_t j0n = P ? ld.jyb : ld.jya, j0s = P ? ld.jsb : ld.jsa;
     const uint64_t j0d = P ? ld.jdb : ld.jda;
 #pragma unroll
@@ -451,7 +456,7 @@ struct fused_ctx {
     }
     ju0 = j0d;
     bs.jw = P ? ld.jxa : ld.jxb;
-    bs.je = P ? ld.jxb : ld.jxn;
+    bs.je = P ? ld.jxb : jxn;
     bs.jn = P ? ld.jya : ld.jyb;
     bs.js = P ? ld.jsa : ld.jsb;
     bs.jd = P ? ld.jda : ld.jdb;
@@ -535,8 +540,8 @@ struct fused_ctx {
     fused_loads<MB> A, B;  // loads of planes z+1 / z+2, used alternately
     bstate bs;
     // prologue: old planes 0 and 1 into LDS, colour-0 update of plane 0
-    load(0, A);
-    load(1, B);
+    load<PT >= 0>(0, A);
+    load<PT >= 0>(1, B);
     put_old<PT>(0, pa, pb);
     put_old<PT>(1, A.na, A.nb);
     __syncthreads();
@@ -555,7 +560,7 @@ struct fused_ctx {
     __syncthreads();
     // iteration z: near = loads of plane z+1, far = loads of plane z+2 (prefetched)
     auto iteration = [&](int z, fused_loads<MB> &near, fused_loads<MB> &far) {
-      load(z + 2 < L ? z + 2 : L - 1, far);  // the last two loads are not used
+      load<PT >= 0>(z + 2 < L ? z + 2 : L - 1, far);  // the last two loads are not used
       nbrs vb;
       read_b<PT>(z, vb);
       {
