@@ -599,7 +599,9 @@ struct fused_ctx {
 // One workgroup's band `tile` of system block `sb` (systems sb*MB .. sb*MB+MB-1):
 // the whole z-walk.  blockDim.x must be (TH+2)*L/2 and the dynamic LDS
 // 3*(TH+2)*MB*2*(L/2)*8 bytes.
-template <bool COUNTER, bool FAST, int MB, int TH>
+// WALK: 1 = rows are whole waves (L/2 a multiple of 64), 0 = they are not, 2 = decide
+// at run time (both walks compiled in).
+template <bool COUNTER, bool FAST, int MB, int TH, int WALK = 2>
 __device__ __forceinline__ void fused_band(const sweep_args &p, uint64_t *__restrict__ s_out, int sb, int tile,
                                            uint64_t *lds) {
   const int L = p.L, PR = L >> 1;
@@ -659,14 +661,16 @@ __device__ __forceinline__ void fused_band(const sweep_args &p, uint64_t *__rest
   // The walk contains barriers, so a parity-specialised walk may only be chosen
   // per wave: when a row is a whole number of waves (L/2 a multiple of 64) every
   // wave has one parity; otherwise the parity stays a run-time value.
-  if ((PR & 63) == 0) {
+  if (WALK == 1 || (WALK == 2 && (PR & 63) == 0)) {
     if (c.par) c.template walk<1>(); else c.template walk<0>();
   } else {
     c.template walk<-1>();
   }
 }
 
-template <bool COUNTER, bool FAST, int MB, int TH>
+// Compiled once per walk (WALK 0 or 1, see fused_band): one kernel holding both
+// walks allocates registers for both and runs slower.
+template <bool COUNTER, bool FAST, int MB, int TH, int WALK>
 __global__ void __launch_bounds__(640) k_sweep3d_fused(sweep_args p, uint64_t *__restrict__ s_out, int xcd_map) {
   extern __shared__ uint64_t lds[];
   const int ntile = p.L / TH;
@@ -685,7 +689,7 @@ __global__ void __launch_bounds__(640) k_sweep3d_fused(sweep_args p, uint64_t *_
     sb = b / ntile;
     tile = b - sb * ntile;
   }
-  fused_band<COUNTER, FAST, MB, TH>(p, s_out, sb, tile, lds);
+  fused_band<COUNTER, FAST, MB, TH, WALK>(p, s_out, sb, tile, lds);
 }
 
 // ---------------------------------------------------------------- 2D half-sweep

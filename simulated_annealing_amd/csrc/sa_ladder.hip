@@ -155,8 +155,12 @@ template <bool COUNTER, bool FAST, int MB, int TH>
 void launch_fused_t(const sa::sweep_args &a, uint64_t *out, const fused_cfg &c, hipStream_t st) {
   const int G = (a.L / TH) * (a.nsys / MB);
   const int threads = (TH + 2) * (a.L / 2);
-  hipLaunchKernelGGL((sa::k_sweep3d_fused<COUNTER, FAST, MB, TH>), dim3(G), dim3(threads),
-                     fused_lds_bytes(a.L, TH, MB), st, a, out, c.xcd);
+  if ((a.L / 2) % 64 == 0)
+    hipLaunchKernelGGL((sa::k_sweep3d_fused<COUNTER, FAST, MB, TH, 1>), dim3(G), dim3(threads),
+                       fused_lds_bytes(a.L, TH, MB), st, a, out, c.xcd);
+  else
+    hipLaunchKernelGGL((sa::k_sweep3d_fused<COUNTER, FAST, MB, TH, 0>), dim3(G), dim3(threads),
+                       fused_lds_bytes(a.L, TH, MB), st, a, out, c.xcd);
 }
 
 template <bool COUNTER, bool FAST>
