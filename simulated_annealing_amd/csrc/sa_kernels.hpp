@@ -768,13 +768,14 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t x, int lane) {
   return x;
 }
 
-template <int KIND>
+template <int KIND, int SB>
 __global__ void __launch_bounds__(256) k_measure(const uint64_t *__restrict__ s_all, const uint64_t *__restrict__ jx,
                                                  const uint64_t *__restrict__ jy, const uint64_t *__restrict__ jz,
                                                  int L, int64_t N, int nsys, const int *__restrict__ sys_list,
                                                  int *cnt) {
+  // SB: systems per pass over the sites (couplings loaded once per pass); with few
+  // systems one per block row gives more waves in flight
   constexpr int FLUSH = 16;  // groups between flushes: E <= 3*16 < 64, M <= 16 < 32
-  constexpr int SB = 4;      // systems per pass over the sites (couplings loaded once per pass)
   __shared__ int red[SB][2][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int sys0 = blockIdx.y * SB;

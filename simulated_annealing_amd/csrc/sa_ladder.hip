@@ -504,23 +504,31 @@ int measure_list(sa_ladder *l, const int *list, int n) {
     SA_HIP_TRY(hipMemcpyAsync(l->sys_list, list, sizeof(int) * n, hipMemcpyHostToDevice, l->stream));
     dlist = l->sys_list;
   }
-  // cnt layout [listed system][2][64]; one block row per 4 systems, ~2 waves/SIMD.
-  // A wave converts its bit-sliced counters (a 64x64 transpose per plane) after 16
-  // site groups or at its end, so each wave gets a full 16 groups where possible:
-  // with few systems more blocks would only add conversions.
-  const int yb = (n + 3) / 4;
-  const int64_t want = std::max<int64_t>(1, 2048 / yb);
+  // cnt layout [listed system][2][64]; one block row per SB systems (4, or 1 when
+  // there are at most 4: more waves in flight for the few-system measurement of an
+  // exchange step).  A wave converts its bit-sliced counters (a 64x64 transpose per
+  // plane) after 16 site groups or at its end, so each wave gets a full 16 groups
+  // where possible: more blocks would only add conversions.
+  const int sb = n <= 4 ? 1 : 4;
+  const int yb = (n + sb - 1) / sb;
+  const int64_t want = std::max<int64_t>(1, (sb == 1 ? 4096 : 2048) / yb);
   const int64_t blocks = std::min<int64_t>((l->N + 256 * 16 - 1) / (256 * 16), want);
   dim3 grid((unsigned)blocks, yb);
-  if (l->kind == SA_KIND_3D)
-    hipLaunchKernelGGL(sa::k_measure<sa::KIND_3D>, grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, l->L,
-                       l->N, n, dlist, l->cnt);
-  else if (l->kind == SA_KIND_2D)
-    hipLaunchKernelGGL(sa::k_measure<sa::KIND_2D>, grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, l->L,
-                       l->N, n, dlist, l->cnt);
-  else
-    hipLaunchKernelGGL(sa::k_measure<sa::KIND_2P>, grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, l->L,
-                       l->N, n, dlist, l->cnt);
+#define SA_MEASURE(KINDV)                                                                                  \
+  if (sb == 1)                                                                                             \
+    hipLaunchKernelGGL((sa::k_measure<KINDV, 1>), grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, \
+                       l->L, l->N, n, dlist, l->cnt);                                                      \
+  else                                                                                                     \
+    hipLaunchKernelGGL((sa::k_measure<KINDV, 4>), grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, \
+                       l->L, l->N, n, dlist, l->cnt);
+  if (l->kind == SA_KIND_3D) {
+    SA_MEASURE(sa::KIND_3D)
+  } else if (l->kind == SA_KIND_2D) {
+    SA_MEASURE(sa::KIND_2D)
+  } else {
+    SA_MEASURE(sa::KIND_2P)
+  }
+#undef SA_MEASURE
   SA_HIP_TRY(hipGetLastError());
   // pageable destination: the copy completes before the synchronisation below, and
   // creating a ladder (e.g. per served request) then needs no pinned allocation
