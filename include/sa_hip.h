@@ -253,7 +253,8 @@ void *sa_batch_stream(sa_batch *b);
 /* tuning (no effect on results): threads per block 64/128/256 of the half-sweep
  * kernel, sweeps per captured graph, and whether 3D instances with symmetric tables
  * and L divisible by 4 take the fused single-pass sweep (1 off, 2 on); 0 = picked
- * online by timing the first chunks */
+ * online by timing the first chunks.  The fused form is switched off in this build:
+ * SA_TUNE_BATCH_FUSED = 2 returns SA_ERR_ARG. */
 enum sa_batch_tune_key { SA_TUNE_BATCH_BLOCK = 101, SA_TUNE_BATCH_CHUNK = 102, SA_TUNE_BATCH_FUSED = 103 };
 int sa_batch_set_tuning(sa_batch *b, int key, int value);
 int sa_batch_get_tuning(sa_batch *b, int key, int *value);
