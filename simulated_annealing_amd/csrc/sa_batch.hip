@@ -13,8 +13,10 @@
 // is captured once per chunk length as a hipGraph and replayed while instances
 // come and go.  Every block runs the per-column / per-site code of the
 // single-instance kernels, so an instance's spins are bit for bit those of its
-// ladder swept alone.  A one-block kernel advances the instances' sweep indices
-// after the second half-sweep.
+// ladder swept alone.  The descriptors hold each instance's sweep index at the
+// start of the chunk; every launch of the chunk gets its sweep's offset as a
+// constant argument, and one one-block kernel at the end of the chunk advances the
+// indices (no index update, and no drain of the whole grid for one, per sweep).
 //
 // Exchanges.  Chunks end on every parallel-tempering instance's exchange points.
 // There the host draws the pair and the uniform of each due instance (its own
@@ -47,41 +49,46 @@ struct inst_desc {
   int pad;
 };
 
+// t: the instance's sweep index (its index at the chunk start plus the sweep's
+// offset in the chunk, a constant of the captured launch)
 template <bool FAST>
-__device__ __forceinline__ void batch_item(const inst_desc &in, int color, int k, int sys) {
+__device__ __forceinline__ void batch_item(const inst_desc &in, int t, int color, int k, int sys) {
   sa::sweep_args a = in.a;
   if (in.cur) a.s = in.s2;  // half-sweeps update the current array in place
-  if (in.per_sweep) a.tab += (int64_t)in.t * a.nsys;
-  a.sweep = (uint64_t)in.t;
+  if (in.per_sweep) a.tab += (int64_t)t * a.nsys;
+  a.sweep = (uint64_t)t;
   if (in.kind == SA_KIND_3D) sa::half3d_column<true, FAST, 1>(a, color, k, sys);
   else if (in.kind == SA_KIND_2D) sa::half2d_site<true, FAST, false, 1>(a, color, k, sys);
   else sa::half2d_site<true, false, true, 1>(a, color, k, sys);
 }
 
-// ctrl[0] = blocks in the block map, ctrl[1] = descriptors
+// ctrl[0] = blocks in the block map, ctrl[1] = descriptors; s = the sweep's offset in
+// the chunk (descriptors hold the sweep index at the chunk start)
 __global__ void __launch_bounds__(256) k_batch_half(const inst_desc *__restrict__ d, const int2 *__restrict__ bmap,
-                                                    const int *__restrict__ ctrl, int color) {
+                                                    const int *__restrict__ ctrl, int color, int s) {
   const int nb = ctrl[0];
   for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
     const int2 bm = bmap[blk];
     const inst_desc &in = d[bm.x];
-    if (in.t >= in.nsweeps) continue;
+    const int t = in.t + s;
+    if (t >= in.nsweeps) continue;
     const int L = in.a.L, work = L * (L >> 1);
     const int item = bm.y + (int)threadIdx.x;
     if (item >= work * in.a.nsys) continue;
     const int sys = item / work, k = item - sys * work;
-    if (in.fast) batch_item<true>(in, color, k, sys);
-    else batch_item<false>(in, color, k, sys);
+    if (in.fast) batch_item<true>(in, t, color, k, sys);
+    else batch_item<false>(in, t, color, k, sys);
   }
 }
 
-__global__ void k_batch_advance(inst_desc *d, const int *__restrict__ ctrl) {
+// end of a chunk of k sweeps: sweep indices advance by the sweeps each instance did
+__global__ void k_batch_advance(inst_desc *d, const int *__restrict__ ctrl, int k) {
   const int n = ctrl[1];
-  for (int i = threadIdx.x; i < n; i += blockDim.x)
-    if (d[i].t < d[i].nsweeps) {
-      d[i].t += 1;
-      if (d[i].fused) d[i].cur ^= 1;  // the fused sweep wrote the other array
-    }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int done = min(k, max(0, d[i].nsweeps - d[i].t));
+    d[i].t += done;
+    if (d[i].fused) d[i].cur ^= done & 1;  // each fused sweep wrote the other array
+  }
 }
 
 // Fused 3D sweeps of one group of instances (same L, band height TH and systems per
@@ -91,18 +98,20 @@ __global__ void k_batch_advance(inst_desc *d, const int *__restrict__ ctrl) {
 constexpr int kMaxFusedGroups = 24;
 template <int MB, int TH>
 __global__ void __launch_bounds__(640) k_batch_fused(const inst_desc *__restrict__ d, const int2 *__restrict__ fmap,
-                                                     const int *__restrict__ ctrl, int g) {
+                                                     const int *__restrict__ ctrl, int g, int s) {
   extern __shared__ uint64_t lds[];
   const int off = ctrl[2 + 2 * g], n = ctrl[3 + 2 * g];
   for (int blk = blockIdx.x; blk < n; blk += gridDim.x) {
     const int2 bm = fmap[off + blk];
     const inst_desc &in = d[bm.x];
-    if (in.t >= in.nsweeps) continue;  // the whole block skips together
+    const int t = in.t + s;
+    if (t >= in.nsweeps) continue;  // the whole block skips together
     sa::sweep_args a = in.a;
-    if (in.per_sweep) a.tab += (int64_t)in.t * a.nsys;
-    a.sweep = (uint64_t)in.t;
-    a.s = in.cur ? in.s2 : in.a.s;
-    uint64_t *out = in.cur ? in.a.s : in.s2;
+    if (in.per_sweep) a.tab += (int64_t)t * a.nsys;
+    a.sweep = (uint64_t)t;
+    const int cur = in.cur ^ (s & 1);  // every earlier sweep of the chunk was fused too
+    a.s = cur ? in.s2 : in.a.s;
+    uint64_t *out = cur ? in.a.s : in.s2;
     const int ntile = a.L / TH;
     const int sb = bm.y / ntile, tile = bm.y - sb * ntile;
     sa::fused_band<true, true, MB, TH>(a, out, sb, tile, lds);
@@ -488,14 +497,14 @@ int rebuild(sa_batch *b, int bs, bool fused) {
   return SA_OK;
 }
 
-void launch_fused_group(sa_batch *b, int g, hipStream_t st) {
+void launch_fused_group(sa_batch *b, int g, int s, hipStream_t st) {
   const sa_batch::fgroup &G = b->graph_groups[g];
   const dim3 grid((unsigned)std::max(1, G.count)), block((unsigned)((G.TH + 2) * (G.L / 2)));
   const size_t lds = (size_t)3 * (G.TH + 2) * G.MB * 2 * (G.L / 2) * 8;
 #define SA_FUSED_CASE(MBV, THV)                                                                               \
   if (G.MB == MBV && G.TH == THV) {                                                                         \
     hipLaunchKernelGGL((k_batch_fused<MBV, THV>), grid, block, lds, st, b->d_desc.p, b->d_fmap.p,            \
-                       b->d_ctrl.p, g);                                                                     \
+                       b->d_ctrl.p, g, s);                                                                  \
     return;                                                                                                 \
   }
   SA_FUSED_CASE(1, 4)
@@ -507,8 +516,8 @@ void launch_fused_group(sa_batch *b, int g, hipStream_t st) {
 #undef SA_FUSED_CASE
 }
 
-// One sweep = the two half-sweep launches and one launch per fused group, then the
-// advance.  The launches touch disjoint instances, so the fused groups go on side
+// One sweep = the two half-sweep launches and one launch per fused group; the
+// chunk ends with the advance.  The launches touch disjoint instances, so the fused groups go on side
 // streams forked from and joined back to the batch stream (parallel branches of
 // the captured graph) and overlap each other and the half-sweeps.
 // side streams and events for ng fused groups (created outside any capture)
@@ -532,15 +541,15 @@ int enqueue_sweeps(sa_batch *b, int k) {
     if (ng) SA_HIP_TRY(hipEventRecord(b->fork, b->stream));
     for (int g = 0; g < ng; ++g) {
       SA_HIP_TRY(hipStreamWaitEvent(b->side[g], b->fork, 0));
-      launch_fused_group(b, g, b->side[g]);
+      launch_fused_group(b, g, s, b->side[g]);
       SA_HIP_TRY(hipEventRecord(b->side_done[g], b->side[g]));
     }
     for (int color = 0; color < 2; ++color)
       hipLaunchKernelGGL(k_batch_half, dim3(grid), dim3(b->bmap_bs), 0, b->stream, b->d_desc.p, b->d_bmap.p,
-                         b->d_ctrl.p, color);
+                         b->d_ctrl.p, color, s);
     for (int g = 0; g < ng; ++g) SA_HIP_TRY(hipStreamWaitEvent(b->stream, b->side_done[g], 0));
-    hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(256), 0, b->stream, b->d_desc.p, b->d_ctrl.p);
   }
+  hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(256), 0, b->stream, b->d_desc.p, b->d_ctrl.p, k);
   return SA_OK;
 }
 
