@@ -170,7 +170,9 @@ class hip_sys : public gpu_sys {
   long get_N() override { return lad_->N(); }
   void swap(gpu_sys *other, std::unique_ptr<spin_t[]> mask) override {
     hip_sys *o = dynamic_cast<hip_sys *>(other);
-    if (!o || o->get_N() != get_N() || o->lad_->kind() != lad_->kind()) {
+    // the reference casts to its own class (cpu_3d / cpu_2dp, src/cpu_3d.cpp:429-444,
+    // src/cpu_2dp.cpp:426-442): any two 2D systems swap (cpu_2d is a cpu_2dp)
+    if (!o || o->get_N() != get_N() || (o->lad_->kind() == SA_KIND_3D) != (lad_->kind() == SA_KIND_3D)) {
       std::cerr << "Error:   conversion error\n";
       return;
     }
@@ -278,12 +280,23 @@ class hip_sys : public gpu_sys {
     return is;
   }
   double get_T() override { return T_; }
-  // an independent copy on its own one-slot ladder (cpu_3d::clone, src/cpu_3d.cpp:494-496)
+  // An independent copy on its own one-slot ladder, as the reference's clone()
+  // makes it (cpu_3d::clone / cpu_2dp::clone, src/cpu_3d.cpp:494-496,
+  // src/cpu_2dp.cpp:481-483, through the copy constructors src/cpu_3d.cpp:47-80,
+  // src/cpu_2dp.cpp:53-86): spins, couplings, T and h are copied, the engine is not
+  // -- the copy's stream restarts from seed 1234 (here the counter stream too, at
+  // sweep 0).  cpu_2d has no clone of its own, so the copy of a 2D periodic system
+  // is a cpu_2dp: open in x.  The copy follows that.
   gpu_sys *clone() const override {
-    hip_sys *c = new hip_sys(lad_->kind(), lad_->L(), T_, h_);
-    std::stringstream ss;
-    const_cast<hip_sys *>(this)->save(ss);
-    c->load(ss);
+    const int kind = lad_->kind() == SA_KIND_2D ? SA_KIND_2P : lad_->kind();
+    hip_sys *c = new hip_sys(kind, lad_->L(), T_, h_);  // a new ladder is seeded with 1234
+    c->set_rng(rng_mode_);
+    c->set_spins(spins());
+    const long N = lad_->N();
+    std::vector<uint64_t> j[3];
+    for (auto &a : j) a.resize(N);
+    check("clone", sa_ladder_get_couplings(l(), j[0].data(), j[1].data(), j[2].data()));
+    check("clone", sa_ladder_set_couplings(c->l(), j[0].data(), j[1].data(), j[2].data()));
     return c;
   }
 

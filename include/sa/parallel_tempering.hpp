@@ -78,6 +78,11 @@ class parallel_tempering {
       // the pair is drawn first (the draws do not depend on the energies), those
       // systems are measured now and the others when E is next read -- their spins
       // do not change before that, so the values are the reference's.
+      // The reference folds E[0] into E_min after every sweep of the step, and E[0]
+      // changes only at the last one: the sweeps before it fold the previous step's
+      // E[0] in again.  That is a no-op except after reset() (E_min zeroed, E[0]
+      // still the energies from before), so it is done once here.
+      if (steps_ > 1) min_E_update();
       sweep_all(steps_);
       i = uniform_int(gen_, uni_a_, uni_b_);
       u = canonical32(gen_);
@@ -122,9 +127,25 @@ class parallel_tempering {
       (*file_E_min_) << info;
     }
   }
+  // src/parallel_tempering.hpp:265-277: every system draws new spins from its own
+  // engine.  The reference's engines all sit at one stream position, so each
+  // system gets the same N words and every engine advances by N; slots of one
+  // device ladder share one engine, so the ladder draws once for all its slots.
   void reset() {
+    // the reference keeps the energies measured at the last step (all systems):
+    // bring the ones not measured then up to date before the spins change
+    refresh();
     for (float &e : E_min_) e = 0.f;
-    for (system *s : sys_) s->init_rand();
+    std::vector<ladder *> done;
+    for (system *s : sys_) {
+      hip_sys *h = as_hip(s);
+      if (h) {
+        ladder *l = h->get_ladder().get();
+        if (std::find(done.begin(), done.end(), l) != done.end()) continue;
+        done.push_back(l);
+      }
+      s->init_rand();
+    }
     for (auto &a : num_switch_) a.reset();
   }
   std::vector<float> get_E_min() const { return E_min_; }
@@ -133,6 +154,8 @@ class parallel_tempering {
     return E_;
   }
   const std::vector<double> &get_T() const { return T_; }
+  // the systems in temperature order (update_T inserts clones)
+  const std::vector<system *> &systems() const { return sys_; }
   std::vector<double> acceptance() const {
     std::vector<double> r;
     for (auto &a : num_switch_) r.push_back(a.get());

@@ -201,6 +201,23 @@ int sa_pt_close_output(sa_pt *pt);
  * a trailer with the counter state.  Loading clears the swap log. */
 int sa_pt_save(sa_pt *pt, const char *path);
 int sa_pt_load(sa_pt *pt, const char *path);
+/* The pieces of main_cpu's ground-state loop (-r, src/main_cpu.cpp:344-434):
+ * sa_pt_update_temperatures = parallel_tempering::update_T
+ * (src/parallel_tempering.hpp:384-413): a temperature (T_i + T_i+1)/2 is inserted
+ * between neighbours whose exchange rate is below 0.3, its system a copy of the
+ * hotter neighbour on a ladder of its own; returns the new temperature count.
+ * sa_pt_reset = parallel_tempering::reset (src/parallel_tempering.hpp:265-277):
+ * E_min cleared, new spins drawn (init_rand), rates kept (the reference's
+ * accumulator::reset does nothing).  sa_pt_temperatures / sa_pt_acceptance copy
+ * up to `capacity` temperatures / exchange rates (num_switch, get()) and return
+ * the count; sa_pt_E_min = get_E_min (64 values, 0 before the first step) and
+ * returns the count.  sa_pt_result's E_last and spins cover every current system
+ * in temperature order. */
+int sa_pt_update_temperatures(sa_pt *pt);
+int sa_pt_reset(sa_pt *pt);
+int sa_pt_temperatures(sa_pt *pt, int capacity, double *T);
+int sa_pt_acceptance(sa_pt *pt, int capacity, double *rates);
+int sa_pt_E_min(sa_pt *pt, float *E_min);
 int sa_pt_destroy(sa_pt *pt);
 
 /* ---- batched instances (no counterpart in the reference) --------------------

@@ -275,6 +275,8 @@ def ref():
                                         C.c_void_p]
         R.ref_pt_files.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, C.c_double, C.c_long, C.c_int, C.c_int, C.c_int,
                                    C.c_char_p, C.c_char_p, C.c_long]
+        R.ref_pt_ground.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, C.c_double, C.c_long, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         R.ref_anneal_geometric.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                            C.c_double, C.c_long, C.c_char_p, C.c_long]
         R.ref_time_sweeps.argtypes = [C.c_int, C.c_int, C.c_double, C.c_long, C.c_int]
@@ -373,6 +375,25 @@ def ref_pt_files(kind, L, T, h, seed, sweeps, nsteps, stride, pattern) -> str:
     n = ref().ref_pt_files(kind, L, T.size, T, h, seed, sweeps, nsteps, stride, str(pattern).encode(), buf, len(buf))
     assert n >= 0
     return buf.value.decode()
+
+
+def ref_pt_ground(kind, L, T, h, seed, sweeps, nsteps, rounds, cap=16):
+    """The reference's ground-state loop pieces on one PT driver (rounds of steps,
+    get_E_min, update_T, reset; then steps once more)."""
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    N = L ** (3 if kind == 2 else 2)
+    nT_out = np.zeros(rounds, dtype=np.int32)
+    acc = np.zeros(rounds * cap, dtype=np.float64)
+    E_min = np.zeros(rounds * 64, dtype=np.float32)
+    T_out = np.zeros(rounds * cap, dtype=np.float64)
+    spins = np.zeros(cap * N, dtype=np.uint64)
+    E = np.zeros(cap * 64, dtype=np.float32)
+    n = ref().ref_pt_ground(kind, L, T.size, T, h, seed, sweeps, nsteps, rounds, cap, nT_out.ctypes.data,
+                            acc.ctypes.data, E_min.ctypes.data, T_out.ctypes.data, spins.ctypes.data, E.ctypes.data)
+    assert n > 0, "over capacity"
+    return dict(nT=nT_out, acc=acc.reshape(rounds, cap), E_min=E_min.reshape(rounds, 64),
+                T_rounds=[T_out[r * cap:r * cap + int(nT_out[r])] for r in range(rounds)],
+                spins=spins.reshape(cap, N)[:n], E=E.reshape(cap, 64)[:n])
 
 
 def ref_anneal_geometric(kind, L, T0, alfa, sweeps, nsteps, stride, h, seed) -> str:

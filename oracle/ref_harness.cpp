@@ -258,6 +258,61 @@ int ref_pt_files(int kind, int L, int nT, const double *Tin, double h, long seed
   return (int)swaps.size();
 }
 
+// The reference's ground-state loop (main_cpu -r, src/main_cpu.cpp:344-434) on one
+// parallel_tempering object: `rounds` times nsteps x step(0), then get_E_min,
+// update_T (a temperature inserted between neighbours whose exchange rate is below
+// 0.3, src/parallel_tempering.hpp:384-413) and reset (E_min cleared, new spins
+// drawn, src/parallel_tempering.hpp:265-277); then nsteps x step(0) once more.
+// Per round: the exchange rates before update_T (acc[r*cap + i], nT-1 of them), the
+// temperature count and temperatures after it (nT_out[r], T_out[r*cap + j]) and
+// E_min (E_min_out[r*64]).  At the end: every system's spins (spins_out[cap*N]) and
+// last energies (E_out[cap*64]).  Returns the final temperature count, -1 if over
+// cap.
+int ref_pt_ground(int kind, int L, int nT, const double *Tin, double h, long seed, int sweeps, int nsteps, int rounds,
+                  int cap, int *nT_out, double *acc, float *E_min_out, double *T_out, uint64_t *spins_out,
+                  float *E_out) {
+  Logger::verbosity = 0;
+  std::vector<cpu_sys *> sys;
+  for (int i = 0; i < nT; ++i) {
+    cpu_sys *q = make_sys(kind, L, Tin[0], h);
+    q->set_seed(seed);
+    q->init_J();
+    q->set_seed(seed);
+    q->init_rand();
+    sys.push_back(q);
+  }
+  std::vector<double> T(Tin, Tin + nT);
+  std::string fn = "ground_%.dat";
+  int n = 0;
+  {
+    parallel_tempering<cpu_sys> pt(sys, T, sweeps, fn);
+    for (int r = 0; r < rounds; ++r) {
+      for (int k = 0; k < nsteps; ++k) pt.step(0);
+      std::vector<float> em = pt.get_E_min();
+      for (int k = 0; k < 64; ++k) E_min_out[r * 64 + k] = k < (int)em.size() ? em[k] : 0.f;
+      for (size_t i = 0; i < pt.num_switch.size() && (int)i < cap; ++i) acc[r * cap + i] = pt.num_switch[i].get();
+      pt.update_T();
+      nT_out[r] = (int)pt.T.size();
+      for (size_t j = 0; j < pt.T.size() && (int)j < cap; ++j) T_out[r * cap + j] = pt.T[j];
+      pt.reset();
+    }
+    for (int k = 0; k < nsteps; ++k) pt.step(0);
+    n = (int)pt.T.size();
+    if (n <= cap) {
+      for (int j = 0; j < n; ++j) {
+        const long N = pt.sys[j]->get_N();
+        if (kind == 2) std::memcpy(spins_out + j * N, static_cast<cpu_3d *>(pt.sys[j])->s, N * 8);
+        else std::memcpy(spins_out + j * N, static_cast<cpu_2dp *>(pt.sys[j])->s, N * 8);
+        for (int k = 0; k < 64; ++k) E_out[j * 64 + k] = k < (int)pt.E[j].size() ? pt.E[j][k] : 0.f;
+      }
+    }
+    for (cpu_sys *q : pt.sys)
+      if (std::find(sys.begin(), sys.end(), q) == sys.end()) delete q;  // the clones update_T made
+  }
+  for (cpu_sys *q : sys) delete q;
+  return n <= cap ? n : -1;
+}
+
 // annealing<cpu_sys> with geometric cooling, exactly the reference driver:
 // annealing(T0, sweeps, alfa, sys, out); nsteps x step(stride, PRINT_E).  Writes the
 // driver's text output (one line of 64 energies per measurement) into buf.

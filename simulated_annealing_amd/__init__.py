@@ -36,7 +36,8 @@ EXPORTS = [
     "sa_pt_run", "sa_anneal_run",
     "sa_pt_create", "sa_pt_step", "sa_pt_sync", "sa_pt_steps_done", "sa_pt_result", "sa_pt_ladder",
     "sa_pt_save", "sa_pt_load", "sa_pt_destroy", "sa_pt_set_output", "sa_pt_step_stride", "sa_pt_print_min",
-    "sa_pt_print_swap", "sa_pt_close_output",
+    "sa_pt_print_swap", "sa_pt_close_output", "sa_pt_update_temperatures", "sa_pt_reset", "sa_pt_temperatures",
+    "sa_pt_acceptance", "sa_pt_E_min",
     "sa_batch_create", "sa_batch_destroy", "sa_batch_add", "sa_batch_add_pt", "sa_batch_run",
     "sa_batch_sweeps_done", "sa_batch_get_spins", "sa_batch_measure", "sa_batch_pt_result", "sa_batch_bytes",
     "sa_batch_ladder", "sa_batch_remove", "sa_batch_stream", "sa_batch_set_tuning", "sa_batch_get_tuning",
@@ -146,6 +147,11 @@ def lib():
         L.sa_pt_save.argtypes = [_vp, C.c_char_p]
         L.sa_pt_load.argtypes = [_vp, C.c_char_p]
         L.sa_pt_destroy.argtypes = [_vp]
+        L.sa_pt_update_temperatures.argtypes = [_vp]
+        L.sa_pt_reset.argtypes = [_vp]
+        L.sa_pt_temperatures.argtypes = [_vp, C.c_int, _vp]
+        L.sa_pt_acceptance.argtypes = [_vp, C.c_int, _vp]
+        L.sa_pt_E_min.argtypes = [_vp, _vp]
         L.sa_batch_create.argtypes = [C.POINTER(_vp)]
         L.sa_batch_destroy.argtypes = [_vp]
         L.sa_batch_add.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, _f64p, C.c_int, C.c_double,
@@ -385,11 +391,47 @@ class PT:
         """parallel_tempering::load from a file (.tempsys); clears the swap log."""
         check(lib().sa_pt_load(self.p, str(path).encode()), "sa_pt_load")
 
+    def update_temperatures(self) -> int:
+        """parallel_tempering::update_T: insert temperatures where the exchange rate
+        is below 0.3; returns the new temperature count."""
+        n = lib().sa_pt_update_temperatures(self.p)
+        check(min(n, 0), "sa_pt_update_temperatures")
+        self.nT = n
+        return n
+
+    def reset(self):
+        """parallel_tempering::reset: E_min cleared, new spins drawn."""
+        check(lib().sa_pt_reset(self.p), "sa_pt_reset")
+
+    def temperatures(self) -> np.ndarray:
+        out = np.zeros(64, dtype=np.float64)
+        n = lib().sa_pt_temperatures(self.p, out.size, _ptr(out))
+        check(min(n, 0), "sa_pt_temperatures")
+        if n > out.size:
+            out = np.zeros(n, dtype=np.float64)
+            lib().sa_pt_temperatures(self.p, out.size, _ptr(out))
+        return out[:n].copy()
+
+    def acceptance(self) -> np.ndarray:
+        """exchange rate of each neighbour pair (num_switch[i].get())"""
+        out = np.zeros(max(self.nT - 1, 1), dtype=np.float64)
+        n = lib().sa_pt_acceptance(self.p, out.size, _ptr(out))
+        check(min(n, 0), "sa_pt_acceptance")
+        return out[:n].copy()
+
+    def E_min(self) -> np.ndarray:
+        out = np.zeros(64, dtype=np.float32)
+        n = lib().sa_pt_E_min(self.p, _ptr(out))
+        check(min(n, 0), "sa_pt_E_min")
+        return out
+
     def result(self, spins=True):
         n = lib().sa_pt_steps_done(self.p)
         pairs = np.zeros(max(n, 1), dtype=np.int32)
         us = np.zeros(max(n, 1), dtype=np.float64)
         masks = np.zeros(max(n, 1), dtype=np.uint64)
+        self.nT = lib().sa_pt_temperatures(self.p, 0, None)  # update_T / load may change it
+        check(min(self.nT, 0), "sa_pt_temperatures")
         E = np.zeros(self.nT * 64, dtype=np.float32)
         sp = np.zeros(self.nT * self.N, dtype=np.uint64) if spins else None
         check(lib().sa_pt_result(self.p, n, _ptr(pairs), _ptr(us), _ptr(masks), _ptr(E), _ptr(sp)), "sa_pt_result")

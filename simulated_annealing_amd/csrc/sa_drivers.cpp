@@ -85,17 +85,53 @@ int sa_pt_result(sa_pt *p, int64_t capacity, int32_t *pairs, double *us, uint64_
     if (masks) masks[k] = p->log[k].mask;
   }
   const auto &E = p->pt->get_E();  // measures systems not measured at the last step
+  const auto &sys = p->pt->systems();  // temperature order, inserted systems included
   if (E_last)
-    for (size_t j = 0; j < p->sys.size(); ++j)
+    for (size_t j = 0; j < sys.size(); ++j)
       for (int k = 0; k < 64; ++k) E_last[j * 64 + k] = E[j].size() == 64 ? E[j][k] : 0.f;
   if (spins) {
     const int64_t N = p->lad->N();
-    for (size_t j = 0; j < p->sys.size(); ++j) {
-      const int rc = sa_ladder_get_spins(p->lad->get(), (int)j, spins + j * N);
-      if (rc != SA_OK) return rc;
+    for (size_t j = 0; j < sys.size(); ++j) {
+      sa::hip_sys *h = dynamic_cast<sa::hip_sys *>(sys[j]);
+      if (!h) return SA_ERR_ARG;
+      const std::vector<uint64_t> v = h->spins();
+      std::memcpy(spins + j * N, v.data(), (size_t)N * 8);
     }
   }
   return SA_OK;
+}
+
+int sa_pt_update_temperatures(sa_pt *p) {
+  if (!p) return SA_ERR_ARG;
+  p->pt->update_T();
+  return (int)p->pt->get_T().size();
+}
+
+int sa_pt_reset(sa_pt *p) {
+  if (!p) return SA_ERR_ARG;
+  p->pt->reset();
+  return SA_OK;
+}
+
+int sa_pt_temperatures(sa_pt *p, int capacity, double *T) {
+  if (!p || capacity < 0 || (capacity && !T)) return SA_ERR_ARG;
+  const auto &v = p->pt->get_T();
+  for (int j = 0; j < capacity && j < (int)v.size(); ++j) T[j] = v[j];
+  return (int)v.size();
+}
+
+int sa_pt_acceptance(sa_pt *p, int capacity, double *rates) {
+  if (!p || capacity < 0 || (capacity && !rates)) return SA_ERR_ARG;
+  const std::vector<double> v = p->pt->acceptance();
+  for (int j = 0; j < capacity && j < (int)v.size(); ++j) rates[j] = v[j];
+  return (int)v.size();
+}
+
+int sa_pt_E_min(sa_pt *p, float *E_min) {
+  if (!p || !E_min) return SA_ERR_ARG;
+  const std::vector<float> v = p->pt->get_E_min();
+  for (int k = 0; k < 64; ++k) E_min[k] = k < (int)v.size() ? v[k] : 0.f;
+  return (int)v.size();
 }
 
 int sa_pt_set_output(sa_pt *p, const char *pattern) {

@@ -5,6 +5,7 @@ sources).  Run in the container that has the reference sources:
     python tests/golden/make_golden.py                 # ref_vectors.json
     python tests/golden/make_golden.py --checkpoints   # ckpt_vectors.json + ckpt_*.tempsys
     python tests/golden/make_golden.py --pt-files      # pt_files.json
+    python tests/golden/make_golden.py --ground        # pt_ground.json
 
 The fixtures are data only: inputs, seeds and the reference's outputs.  Spin arrays
 are stored whole (hex words) for small lattices and as SHA-256 of their
@@ -26,6 +27,7 @@ import oracle as O  # noqa: E402
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.json")
 CKPT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ckpt_vectors.json")
 PTF_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pt_files.json")
+GROUND_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pt_ground.json")
 
 
 def hexw(a) -> list[str]:
@@ -177,9 +179,32 @@ def pt_files():
     return out
 
 
+def ground():
+    """The ground-state loop pieces of main_cpu -r on one reference PT driver: rounds
+    of steps, get_E_min, update_T, reset, then steps once more (replay stream)."""
+    out = []
+    for kind, L, T, h, seed, sweeps, nsteps, rounds in [(2, 8, [0.4, 1.2, 3.0], 0.0, 77, 5, 20, 2),
+                                                         (1, 8, [0.3, 1.0, 2.5, 4.0], 0.2, 78, 4, 15, 2),
+                                                         (0, 10, [0.5, 2.0, 5.0], 0.0, 79, 3, 12, 3)]:
+        r = O.ref_pt_ground(kind, L, np.array(T), h, seed, sweeps, nsteps, rounds)
+        out.append(dict(kind=kind, L=L, T=T, h=h, seed=seed, sweeps=sweeps, nsteps=nsteps, rounds=rounds,
+                        nT=[int(x) for x in r["nT"]],
+                        acc=[f64bits(r["acc"][q][:(T.__len__() if q == 0 else int(r["nT"][q - 1])) - 1])
+                             for q in range(rounds)],
+                        T_rounds=[f64bits(t) for t in r["T_rounds"]], E_min=[f32bits(e) for e in r["E_min"]],
+                        spins_sha=[sha(x) for x in r["spins"]], E=[f32bits(e) for e in r["E"]]))
+    return out
+
+
 def main():
     if not O.ref_available():
         O.build_ref()
+    if "--ground" in sys.argv:  # only the ground-state loop fixtures
+        with open(GROUND_OUT, "w") as f:
+            json.dump(dict(about="update_T / reset / get_E_min of the reference PT driver (see make_golden.py).",
+                           runs=ground()), f, indent=0, separators=(",", ":"))
+        print("wrote", GROUND_OUT)
+        return
     if "--pt-files" in sys.argv:  # only the PT output-file fixtures
         with open(PTF_OUT, "w") as f:
             json.dump(dict(about="parallel_tempering output files of the reference (see make_golden.py).",
