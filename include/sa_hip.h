@@ -167,6 +167,15 @@ int sa_pt_run(int kind, int L, int nT, const double *T, double h, uint64_t seed,
  * sa_status. */
 int sa_anneal_run(int kind, int L, const double *Ts, int nsweeps, double h, uint64_t seed, int stride,
                   int rng_mode, float *E_log, uint64_t *spins_out);
+/* The annealing driver with geometric cooling as main runs it
+ * (annealing<system>(T0, sweeps, alfa, sys, out); nsteps x step(stride, PRINT_E),
+ * src/annealing.hpp:42-100, src/main_gpu.cu:279-300): each step is `sweeps` sweeps
+ * at the current T, a line of 64 energies every `stride` sweeps, then T *= alfa.
+ * The system is seeded like main.  Returns the text's length and copies it
+ * (NUL-terminated, truncated to capacity) when buf is not NULL; `precision` is the
+ * stream precision (0: the stream default, as main's output file). */
+int64_t sa_anneal_geometric(int kind, int L, double T0, double alfa, int sweeps, int nsteps, int stride, double h,
+                            uint64_t seed, int rng_mode, int precision, char *buf, int64_t capacity);
 
 /* A parallel-tempering run kept alive between calls (what sa_pt_run does in one
  * call), for callers that time or interleave steps: create seeds the systems,

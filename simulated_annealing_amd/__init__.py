@@ -33,7 +33,7 @@ EXPORTS = [
     "sa_ladder_anneal",
     "sa_ladder_measure", "sa_ladder_measure_systems", "sa_ladder_swap", "sa_ladder_sync", "sa_ladder_stream",
     "sa_ladder_set_tuning", "sa_ladder_get_tuning",
-    "sa_pt_run", "sa_anneal_run",
+    "sa_pt_run", "sa_anneal_run", "sa_anneal_geometric",
     "sa_pt_create", "sa_pt_step", "sa_pt_sync", "sa_pt_steps_done", "sa_pt_result", "sa_pt_ladder",
     "sa_pt_save", "sa_pt_load", "sa_pt_destroy", "sa_pt_set_output", "sa_pt_step_stride", "sa_pt_print_min",
     "sa_pt_print_swap", "sa_pt_close_output", "sa_pt_update_temperatures", "sa_pt_reset", "sa_pt_temperatures",
@@ -129,6 +129,9 @@ def lib():
                                 _vp, _vp, _vp, _vp, _vp]
         L.sa_anneal_run.argtypes = [C.c_int, C.c_int, _f64p, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int,
                                     _vp, _vp]
+        L.sa_anneal_geometric.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                          C.c_double, C.c_uint64, C.c_int, C.c_int, C.c_char_p, C.c_int64]
+        L.sa_anneal_geometric.restype = C.c_int64
         L.sa_pt_create.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, C.c_double, C.c_uint64, C.c_int, C.c_int,
                                    C.POINTER(_vp)]
         L.sa_pt_step.argtypes = [_vp, C.c_int]
@@ -447,6 +450,17 @@ class PT:
 
     def __del__(self):
         self.close()
+
+
+def anneal_geometric(kind, L, T0, alfa, sweeps, nsteps, stride, h=0.0, seed=1234, rng_mode=RNG_COUNTER,
+                     precision=0) -> str:
+    """annealing<system>::step with geometric cooling (sa_anneal_geometric): the
+    driver's printed energy lines."""
+    n = lib().sa_anneal_geometric(kind, L, T0, alfa, sweeps, nsteps, stride, h, seed, rng_mode, precision, None, 0)
+    check(int(min(n, 0)), "sa_anneal_geometric")
+    buf = C.create_string_buffer(int(n) + 1)
+    lib().sa_anneal_geometric(kind, L, T0, alfa, sweeps, nsteps, stride, h, seed, rng_mode, precision, buf, len(buf))
+    return buf.value.decode()
 
 
 def linear_schedule(T0: float, T1: float, nsweeps: int) -> np.ndarray:

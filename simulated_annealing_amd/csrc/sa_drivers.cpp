@@ -232,4 +232,32 @@ int sa_anneal_run(int kind, int L, const double *Ts, int nsweeps, double h, uint
   return (int)log.size();
 }
 
+int64_t sa_anneal_geometric(int kind, int L, double T0, double alfa, int sweeps, int nsteps, int stride, double h,
+                            uint64_t seed, int rng_mode, int precision, char *buf, int64_t capacity) {
+  if (sweeps < 1 || nsteps < 0 || stride < 0 || alfa < 0 || alfa > 1 || precision < 0 || capacity < 0 ||
+      (capacity && !buf))
+    return SA_ERR_ARG;
+  std::unique_ptr<sa::hip_sys> s;
+  try {
+    s.reset(new sa::hip_sys(kind, L, T0, h));
+  } catch (const std::exception &) {
+    return SA_ERR_NOMEM;
+  }
+  s->set_rng(rng_mode);
+  seed_like_main(*s, seed);
+  std::ostringstream os;
+  if (precision) os.precision(precision);
+  {
+    sa::annealing<sa::hip_sys> a(T0, sweeps, alfa, s.get(), os);
+    for (int k = 0; k < nsteps; ++k) a.step(stride, sa::PRINT_E);
+  }
+  const std::string t = os.str();
+  if (capacity) {
+    const int64_t n = std::min<int64_t>((int64_t)t.size(), capacity - 1);
+    std::memcpy(buf, t.data(), (size_t)n);
+    buf[n] = '\0';
+  }
+  return (int64_t)t.size();
+}
+
 }  // extern "C"

@@ -344,3 +344,14 @@ def test_ladders_swept_from_concurrent_threads(sa):
     for q in range(len(cases)):
         for a, b in zip(seq[q], par[q]):
             assert (a == b).all(), cases[q]
+
+
+def test_geometric_anneal_driver_text_matches_reference(sa, golden):
+    """annealing<system>::step with geometric cooling on the device (sa_anneal_geometric),
+    replay stream, against the compiled reference driver's printed lines
+    (src/annealing.hpp:67-100): byte for byte at the fixture's stream precision (9)."""
+    for c in golden["anneal"]:
+        say("geometric anneal", c["kind"], c["L"], c["T0"], c["alfa"])
+        txt = sa.anneal_geometric(c["kind"], c["L"], c["T0"], c["alfa"], c["sweeps"], c["nsteps"], c["stride"],
+                                  0.0, c["seed"], sa.RNG_REPLAY, precision=9)
+        assert txt == c["text"]
