@@ -208,7 +208,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(independent ladders shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+              "(independent ladders and batch instances shard across GPUs without communication: one process "
+              "per GPU, each with its own share)", file=sys.stderr)
         sys.exit(2)
 
     import simulated_annealing_amd as sa
