@@ -39,7 +39,9 @@ struct sa_ladder {
   uint64_t *words_d = nullptr, *words_h = nullptr;
   int64_t words_cap = 0;  // sweeps that fit in the staging buffers
   std::vector<int> cnt_h;  // host copy of the measurement counters
-  int *sys_list = nullptr;  // device copy of a measured subset
+  int *cnt_pin = nullptr;  // pinned landing buffer of the counters (allocated at the first measurement)
+  hipEvent_t meas_ev = nullptr;  // recorded after the counters' copy; polled
+  int *sys_list = nullptr;  // device copy of a measured subset (more than 4 systems)
   int sys_per_thread = 0;  // SA_TUNE_SYS_PER_THREAD, 0 = automatic
   int sweep3d_kernel = 0;  // SA_TUNE_SWEEP3D_KERNEL, 0 = automatic
   int rows_per_tile = 0;   // SA_TUNE_ROWS_PER_TILE, 0 = automatic

@@ -768,11 +768,16 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t x, int lane) {
   return x;
 }
 
+// up to four listed systems, passed by value
+struct sys4 {
+  int v[4];
+};
+
 template <int KIND, int SB>
 __global__ void __launch_bounds__(256) k_measure(const uint64_t *__restrict__ s_all, const uint64_t *__restrict__ jx,
                                                  const uint64_t *__restrict__ jy, const uint64_t *__restrict__ jz,
                                                  int L, int64_t N, int nsys, const int *__restrict__ sys_list,
-                                                 int *cnt) {
+                                                 sys4 sel, int *cnt) {
   // SB: systems per pass over the sites (couplings loaded once per pass); with few
   // systems one per block row gives more waves in flight
   constexpr int FLUSH = 16;  // groups between flushes: E <= 3*16 < 64, M <= 16 < 32
@@ -842,7 +847,8 @@ __global__ void __launch_bounds__(256) k_measure(const uint64_t *__restrict__ s_
 #pragma unroll
     for (int m = 0; m < SB; ++m) {
       if (m >= ns) break;
-      const int sys = sys_list ? sys_list[sys0 + m] : sys0 + m;
+      const int q = sys0 + m;
+      const int sys = sys_list ? sys_list[q] : (sel.v[0] >= 0 ? sel.v[q & 3] : q);  // sel: at most 4 systems
       const uint64_t *s = s_all + (int64_t)sys * N;
       uint64_t w = 0, bw = 0, bn = 0, bu = 0;
       if (in) {

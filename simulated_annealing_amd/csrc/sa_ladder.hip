@@ -42,6 +42,8 @@ void free_all(sa_ladder *l) {
   if (l->words_d) (void)hipFree(l->words_d);
   if (l->words_h) (void)hipHostFree(l->words_h);
   if (l->sys_list) (void)hipFree(l->sys_list);
+  if (l->cnt_pin) (void)hipHostFree(l->cnt_pin);
+  if (l->meas_ev) (void)hipEventDestroy(l->meas_ev);
   if (l->stream) (void)hipStreamDestroy(l->stream);
 }
 
@@ -499,8 +501,12 @@ namespace {
 int measure_list(sa_ladder *l, const int *list, int n) {
   const int cnt_n = 128 * n;
   hipLaunchKernelGGL(sa::k_fill_zero, dim3((cnt_n + 255) / 256), dim3(256), 0, l->stream, l->cnt, cnt_n);
+  // up to 4 listed systems travel as a kernel argument; more through device memory
   const int *dlist = nullptr;
-  if (list) {
+  sa::sys4 sel{{-1, -1, -1, -1}};
+  if (list && n <= 4) {
+    for (int j = 0; j < n; ++j) sel.v[j] = list[j];
+  } else if (list) {
     SA_HIP_TRY(hipMemcpyAsync(l->sys_list, list, sizeof(int) * n, hipMemcpyHostToDevice, l->stream));
     dlist = l->sys_list;
   }
@@ -517,10 +523,10 @@ int measure_list(sa_ladder *l, const int *list, int n) {
 #define SA_MEASURE(KINDV)                                                                                  \
   if (sb == 1)                                                                                             \
     hipLaunchKernelGGL((sa::k_measure<KINDV, 1>), grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, \
-                       l->L, l->N, n, dlist, l->cnt);                                                      \
+                       l->L, l->N, n, dlist, sel, l->cnt);                                                      \
   else                                                                                                     \
     hipLaunchKernelGGL((sa::k_measure<KINDV, 4>), grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, \
-                       l->L, l->N, n, dlist, l->cnt);
+                       l->L, l->N, n, dlist, sel, l->cnt);
   if (l->kind == SA_KIND_3D) {
     SA_MEASURE(sa::KIND_3D)
   } else if (l->kind == SA_KIND_2D) {
@@ -530,10 +536,25 @@ int measure_list(sa_ladder *l, const int *list, int n) {
   }
 #undef SA_MEASURE
   SA_HIP_TRY(hipGetLastError());
-  // pageable destination: the copy completes before the synchronisation below, and
-  // creating a ladder (e.g. per served request) then needs no pinned allocation
-  SA_HIP_TRY(hipMemcpyAsync(l->cnt_h.data(), l->cnt, sizeof(int) * cnt_n, hipMemcpyDeviceToHost, l->stream));
-  SA_HIP_TRY(hipStreamSynchronize(l->stream));
+  // The counters land in pinned memory allocated at the first measurement (so
+  // creating a ladder, e.g. per served request, needs no pinned allocation) and the
+  // host polls an event instead of sleeping in a stream synchronisation: the
+  // replica-exchange step waits on this between sweeps.
+  if (!l->cnt_pin && hipHostMalloc((void **)&l->cnt_pin, sizeof(int) * 128 * l->nsys, hipHostMallocDefault) != hipSuccess)
+    l->cnt_pin = nullptr;
+  if (!l->meas_ev && hipEventCreateWithFlags(&l->meas_ev, hipEventDisableTiming) != hipSuccess) l->meas_ev = nullptr;
+  if (l->cnt_pin && l->meas_ev) {
+    SA_HIP_TRY(hipMemcpyAsync(l->cnt_pin, l->cnt, sizeof(int) * cnt_n, hipMemcpyDeviceToHost, l->stream));
+    SA_HIP_TRY(hipEventRecord(l->meas_ev, l->stream));
+    hipError_t e;
+    while ((e = hipEventQuery(l->meas_ev)) == hipErrorNotReady) {
+    }
+    SA_HIP_TRY(e);
+    std::memcpy(l->cnt_h.data(), l->cnt_pin, sizeof(int) * cnt_n);
+  } else {
+    SA_HIP_TRY(hipMemcpyAsync(l->cnt_h.data(), l->cnt, sizeof(int) * cnt_n, hipMemcpyDeviceToHost, l->stream));
+    SA_HIP_TRY(hipStreamSynchronize(l->stream));
+  }
   return SA_OK;
 }
 
