@@ -768,6 +768,11 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t x, int lane) {
   return x;
 }
 
+// The measurement's per-lane counters are kept in kMeasureCopies copies
+// (cnt[copy][listed system][128], copy = block index mod kMeasureCopies), summed on
+// the host: blocks then contend for an address 8 times less.
+constexpr int kMeasureCopies = 8;
+
 // up to four listed systems, passed by value
 struct sys4 {
   int v[4];
@@ -815,71 +820,90 @@ __global__ void __launch_bounds__(256) k_measure(const uint64_t *__restrict__ s_
       }
     }
   };
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + wv * 64; base < N; base += stride) {
-    // coordinates of the chunk's first site once per wave (scalar), then per lane
-    // by stepping along the row: no per-site division
-    const uint32_t b32 = (uint32_t)base, uL = (uint32_t)L, uL2 = (uint32_t)L2;
-    const uint32_t zb = b32 / uL2, rem = b32 - zb * uL2, yb = rem / uL, xb = rem - yb * uL;
-    uint32_t x = xb + (uint32_t)lane, y = yb, z = zb;
-    while (x >= uL) {  // at most 64/L steps
-      x -= uL;
-      if (++y == uL) {
-        y = 0;
-        ++z;
-      }
-    }
-    const uint32_t i = b32 + (uint32_t)lane;
-    const bool in = (int64_t)i < N;
-    uint32_t iw = 0, inn = 0, iu = 0;
-    uint64_t Jw = 0, Jn = 0, Ju = 0;
-    bool west = true;
-    if (in) {
-      iw = i - x + (x == 0 ? uL - 1 : x - 1);
-      inn = (y == 0) ? i + uL2 - uL : i - uL;
-      Jw = jx[i];
-      Jn = jy[i];
-      west = !(KIND == KIND_2P && x == 0);  // no open-x west bond
-      if (KIND == KIND_3D) {
-        iu = (z == 0) ? i + (uint32_t)(N - L2) : i - uL2;
-        Ju = jz[i];
-      }
-    }
+  // U groups per pass: all their loads are issued before any of them is counted, so
+  // a wave waits for memory once per pass (with one system per block row; with four
+  // the loads of one group already fill the registers)
+  constexpr int U = SB == 1 ? 4 : 1;
+  uint64_t Jw[U], Jn[U], Ju[U], w[U][SB], sw[U][SB], sn[U][SB], su[U][SB];
+  bool west[U];
+  for (int64_t base0 = (int64_t)blockIdx.x * blockDim.x + wv * 64; base0 < N; base0 += U * stride) {
 #pragma unroll
-    for (int m = 0; m < SB; ++m) {
-      if (m >= ns) break;
-      const int q = sys0 + m;
-      const int sys = sys_list ? sys_list[q] : (sel.v[0] >= 0 ? sel.v[q & 3] : q);  // sel: at most 4 systems
-      const uint64_t *s = s_all + (int64_t)sys * N;
-      uint64_t w = 0, bw = 0, bn = 0, bu = 0;
+    for (int g = 0; g < U; ++g) {
+      const int64_t base = base0 + g * stride;
+      // coordinates of the group's first site once per wave (scalar), then per lane
+      // by stepping along the row: no per-site division
+      const uint32_t b32 = (uint32_t)base, uL = (uint32_t)L, uL2 = (uint32_t)L2;
+      const uint32_t zb = b32 / uL2, rem = b32 - zb * uL2, yb = rem / uL, xb = rem - yb * uL;
+      uint32_t x = xb + (uint32_t)lane, y = yb, z = zb;
+      while (x >= uL) {  // at most 64/L steps
+        x -= uL;
+        if (++y == uL) {
+          y = 0;
+          ++z;
+        }
+      }
+      const uint32_t i = b32 + (uint32_t)lane;
+      const bool in = base < N && (int64_t)i < N;
+      Jw[g] = Jn[g] = Ju[g] = 0;
+      west[g] = true;
+#pragma unroll
+      for (int m = 0; m < SB; ++m) w[g][m] = sw[g][m] = sn[g][m] = su[g][m] = 0;
       if (in) {
-        w = s[i];
-        bw = west ? (Jw ^ w ^ s[iw]) : 0;
-        bn = Jn ^ w ^ s[inn];
-        if (KIND == KIND_3D) bu = Ju ^ w ^ s[iu];
-      }
-      // satisfied bonds of this site: sum bit sb (weight 1) and carry cb (weight 2)
-      const uint64_t sb = bw ^ bn ^ bu, cb = maj3(bw, bn, bu);
-      uint64_t t = e[m][0] & sb;
-      e[m][0] ^= sb;
-      uint64_t u = maj3(e[m][1], cb, t);
-      e[m][1] ^= cb ^ t;
+        const uint32_t iw = i - x + (x == 0 ? uL - 1 : x - 1);
+        const uint32_t inn = (y == 0) ? i + uL2 - uL : i - uL;
+        const uint32_t iu = (KIND == KIND_3D) ? ((z == 0) ? i + (uint32_t)(N - L2) : i - uL2) : 0;
+        Jw[g] = jx[i];
+        Jn[g] = jy[i];
+        west[g] = !(KIND == KIND_2P && x == 0);  // no open-x west bond
+        if (KIND == KIND_3D) Ju[g] = jz[i];
 #pragma unroll
-      for (int b = 2; b < 6; ++b) {
-        const uint64_t v = e[m][b] & u;
-        e[m][b] ^= u;
-        u = v;
-      }
-      t = w;
-#pragma unroll
-      for (int b = 0; b < 5; ++b) {
-        const uint64_t v = mc[m][b] & t;
-        mc[m][b] ^= t;
-        t = v;
+        for (int m = 0; m < SB; ++m) {
+          if (m >= ns) break;
+          const int q = sys0 + m;
+          const int sys = sys_list ? sys_list[q] : (sel.v[0] >= 0 ? sel.v[q & 3] : q);  // sel: at most 4 systems
+          const uint64_t *s = s_all + (int64_t)sys * N;
+          w[g][m] = s[i];
+          sw[g][m] = s[iw];
+          sn[g][m] = s[inn];
+          if (KIND == KIND_3D) su[g][m] = s[iu];
+        }
       }
     }
-    if (++pending == FLUSH) {
-      flush();
-      pending = 0;
+#pragma unroll
+    for (int g = 0; g < U; ++g) {
+      if (base0 + g * stride >= N) break;  // wave-uniform
+#pragma unroll
+      for (int m = 0; m < SB; ++m) {
+        if (m >= ns) break;
+        const uint64_t ww = w[g][m];
+        const uint64_t bw = west[g] ? (Jw[g] ^ ww ^ sw[g][m]) : 0;
+        const uint64_t bn = Jn[g] ^ ww ^ sn[g][m];
+        const uint64_t bu = (KIND == KIND_3D) ? (Ju[g] ^ ww ^ su[g][m]) : 0;
+        // satisfied bonds of this site: sum bit sb (weight 1) and carry cb (weight 2);
+        // lanes outside the lattice hold zero words and count nothing
+        const uint64_t sb = bw ^ bn ^ bu, cb = maj3(bw, bn, bu);
+        uint64_t t = e[m][0] & sb;
+        e[m][0] ^= sb;
+        uint64_t u = maj3(e[m][1], cb, t);
+        e[m][1] ^= cb ^ t;
+#pragma unroll
+        for (int b = 2; b < 6; ++b) {
+          const uint64_t v = e[m][b] & u;
+          e[m][b] ^= u;
+          u = v;
+        }
+        t = ww;
+#pragma unroll
+        for (int b = 0; b < 5; ++b) {
+          const uint64_t v = mc[m][b] & t;
+          mc[m][b] ^= t;
+          t = v;
+        }
+      }
+      if (++pending == FLUSH) {
+        flush();
+        pending = 0;
+      }
     }
   }
   if (pending) flush();
@@ -890,7 +914,7 @@ __global__ void __launch_bounds__(256) k_measure(const uint64_t *__restrict__ s_
   }
   __syncthreads();
   for (int q = threadIdx.x; q < ns * 128; q += blockDim.x)
-    atomicAdd(&cnt[(int64_t)sys0 * 128 + q], (&red[0][0][0])[q]);
+    atomicAdd(&cnt[((int64_t)(blockIdx.x % kMeasureCopies) * nsys + sys0) * 128 + q], (&red[0][0][0])[q]);
 }
 
 static __global__ void k_fill_zero(int *p, int n) {

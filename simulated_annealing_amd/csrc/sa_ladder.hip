@@ -268,7 +268,7 @@ int sa_ladder_create(int kind, int L, int nsys, sa_ladder **out) {
   if (bad(hipGetDevice(&l->device), "hipGetDevice") || bad(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking), "hipStreamCreate") ||
       bad(hipMalloc(&l->s, (size_t)nsys * N * 8), "spins") || bad(hipMalloc(&l->jx, 3 * N * 8), "couplings") ||
       bad(hipMalloc(&l->tab, sizeof(sa::sys_table) * nsys), "tables") ||
-      bad(hipMalloc(&l->cnt, sizeof(int) * 128 * nsys), "counters") ||
+      bad(hipMalloc(&l->cnt, sizeof(int) * 128 * nsys * sa::kMeasureCopies), "counters") ||
       bad(hipMalloc(&l->sys_list, sizeof(int) * nsys), "system list") ||
       bad(hipMemsetAsync(l->s, 0, (size_t)nsys * N * 8, l->stream), "memset") ||
       bad(hipMemsetAsync(l->jx, 0, 3 * N * 8, l->stream), "memset")) {
@@ -280,7 +280,7 @@ int sa_ladder_create(int kind, int L, int nsys, sa_ladder **out) {
   l->jz = l->jx + 2 * N;
   l->htab.assign(nsys, sa::sys_table{});
   l->fast.assign(nsys, 0);
-  l->cnt_h.assign((size_t)128 * nsys, 0);
+  l->cnt_h.assign((size_t)128 * nsys * sa::kMeasureCopies, 0);
   l->eng.seed(5489u);
   *out = l;
   for (int i = 0; i < nsys; ++i) {
@@ -420,7 +420,8 @@ int sa_ladder_get_spins(const sa_ladder *l, int sys, uint64_t *s) {
 
 int64_t sa_detail::ladder_bytes(int kind, int L, int nsys) {
   const int64_t N = (int64_t)L * L * (kind == SA_KIND_3D ? L : 1);
-  return (int64_t)nsys * N * 8 + 3 * N * 8 + (int64_t)sizeof(sa::sys_table) * nsys + 4 * 128 * nsys + 4 * nsys;
+  return (int64_t)nsys * N * 8 + 3 * N * 8 + (int64_t)sizeof(sa::sys_table) * nsys +
+         4 * 128 * nsys * sa::kMeasureCopies + 4 * nsys;
 }
 
 void sa_detail::make_table(int kind, double T, double h, sa::sys_table &t, bool &fast, double *boltz_out) {
@@ -499,8 +500,8 @@ namespace {
 // energies/magnetisations of `n` systems (list = nullptr: systems 0..n-1); results
 // land in cnt_h[(2*j)*64 ...] for the j-th listed system
 int measure_list(sa_ladder *l, const int *list, int n) {
-  const int cnt_n = 128 * n;
-  hipLaunchKernelGGL(sa::k_fill_zero, dim3((cnt_n + 255) / 256), dim3(256), 0, l->stream, l->cnt, cnt_n);
+  const int cnt_n = 128 * n, all_n = cnt_n * sa::kMeasureCopies;
+  hipLaunchKernelGGL(sa::k_fill_zero, dim3((all_n + 255) / 256), dim3(256), 0, l->stream, l->cnt, all_n);
   // up to 4 listed systems travel as a kernel argument; more through device memory
   const int *dlist = nullptr;
   sa::sys4 sel{{-1, -1, -1, -1}};
@@ -510,15 +511,17 @@ int measure_list(sa_ladder *l, const int *list, int n) {
     SA_HIP_TRY(hipMemcpyAsync(l->sys_list, list, sizeof(int) * n, hipMemcpyHostToDevice, l->stream));
     dlist = l->sys_list;
   }
-  // cnt layout [listed system][2][64]; one block row per SB systems (4, or 1 when
-  // there are at most 4: more waves in flight for the few-system measurement of an
-  // exchange step).  A wave converts its bit-sliced counters (a 64x64 transpose per
-  // plane) after 16 site groups or at its end, so each wave gets a full 16 groups
-  // where possible: more blocks would only add conversions.
+  // cnt layout [copy][listed system][2][64] (the copies are summed below); one block
+  // row per SB systems (4, or 1 when there are at most 4: more waves in flight for
+  // the few-system measurement of an exchange step).  A wave converts its bit-sliced
+  // counters (a 64x64 transpose per plane) after 16 site groups or at its end, so
+  // each wave gets at least 16 groups where possible (32 with one system per row,
+  // four of them loaded at a time): more blocks would only add conversions.
   const int sb = n <= 4 ? 1 : 4;
   const int yb = (n + sb - 1) / sb;
   const int64_t want = std::max<int64_t>(1, (sb == 1 ? 4096 : 2048) / yb);
-  const int64_t blocks = std::min<int64_t>((l->N + 256 * 16 - 1) / (256 * 16), want);
+  const int64_t per = sb == 1 ? 32 : 16;  // site groups per wave (one system per row: 4 in flight at a time)
+  const int64_t blocks = std::min<int64_t>((l->N + 256 * per - 1) / (256 * per), want);
   dim3 grid((unsigned)blocks, yb);
 #define SA_MEASURE(KINDV)                                                                                  \
   if (sb == 1)                                                                                             \
@@ -540,21 +543,26 @@ int measure_list(sa_ladder *l, const int *list, int n) {
   // creating a ladder, e.g. per served request, needs no pinned allocation) and the
   // host polls an event instead of sleeping in a stream synchronisation: the
   // replica-exchange step waits on this between sweeps.
-  if (!l->cnt_pin && hipHostMalloc((void **)&l->cnt_pin, sizeof(int) * 128 * l->nsys, hipHostMallocDefault) != hipSuccess)
+  if (!l->cnt_pin &&
+      hipHostMalloc((void **)&l->cnt_pin, sizeof(int) * 128 * l->nsys * sa::kMeasureCopies, hipHostMallocDefault) !=
+          hipSuccess)
     l->cnt_pin = nullptr;
   if (!l->meas_ev && hipEventCreateWithFlags(&l->meas_ev, hipEventDisableTiming) != hipSuccess) l->meas_ev = nullptr;
   if (l->cnt_pin && l->meas_ev) {
-    SA_HIP_TRY(hipMemcpyAsync(l->cnt_pin, l->cnt, sizeof(int) * cnt_n, hipMemcpyDeviceToHost, l->stream));
+    SA_HIP_TRY(hipMemcpyAsync(l->cnt_pin, l->cnt, sizeof(int) * all_n, hipMemcpyDeviceToHost, l->stream));
     SA_HIP_TRY(hipEventRecord(l->meas_ev, l->stream));
     hipError_t e;
     while ((e = hipEventQuery(l->meas_ev)) == hipErrorNotReady) {
     }
     SA_HIP_TRY(e);
-    std::memcpy(l->cnt_h.data(), l->cnt_pin, sizeof(int) * cnt_n);
+    std::memcpy(l->cnt_h.data(), l->cnt_pin, sizeof(int) * all_n);
   } else {
-    SA_HIP_TRY(hipMemcpyAsync(l->cnt_h.data(), l->cnt, sizeof(int) * cnt_n, hipMemcpyDeviceToHost, l->stream));
+    SA_HIP_TRY(hipMemcpyAsync(l->cnt_h.data(), l->cnt, sizeof(int) * all_n, hipMemcpyDeviceToHost, l->stream));
     SA_HIP_TRY(hipStreamSynchronize(l->stream));
   }
+  // sum the copies into the first
+  for (int c = 1; c < sa::kMeasureCopies; ++c)
+    for (int q = 0; q < cnt_n; ++q) l->cnt_h[q] += l->cnt_h[(size_t)c * cnt_n + q];
   return SA_OK;
 }
 
