@@ -57,7 +57,7 @@ __device__ __forceinline__ void batch_item(const inst_desc &in, int t, int color
   if (in.cur) a.s = in.s2;  // half-sweeps update the current array in place
   if (in.per_sweep) a.tab += (int64_t)t * a.nsys;
   a.sweep = (uint64_t)t;
-  if (in.kind == SA_KIND_3D) sa::half3d_column<true, FAST, 1>(a, color, k, sys);
+  if (in.kind == SA_KIND_3D) sa::half3d_column<true, FAST, 1, true>(a, color, k, sys);
   else if (in.kind == SA_KIND_2D) sa::half2d_site<true, FAST, false, 1>(a, color, k, sys);
   else sa::half2d_site<true, false, true, 1>(a, color, k, sys);
 }

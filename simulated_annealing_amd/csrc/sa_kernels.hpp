@@ -209,7 +209,9 @@ struct plane_loads {
 };
 
 // column k (0 <= k < L*L/2) of colour `color`, systems sys0 .. sys0+MB-1
-template <bool COUNTER, bool FAST, int MB>
+// DEEP: plane loads issued two planes ahead instead of one (more registers; pays
+// where few long column walks set the pace, as in the batched engine)
+template <bool COUNTER, bool FAST, int MB, bool DEEP = false>
 __device__ __forceinline__ void half3d_column(const sweep_args &p, int color, int k, int sys0) {
   const int L = p.L;
   const int half = L >> 1;
@@ -254,10 +256,10 @@ __device__ __forceinline__ void half3d_column(const sweep_args &p, int color, in
   };
   uint64_t jz_up = p.jz[c0];
   u32x4 blk{0, 0, 0, 0};
-  plane_loads<MB> cur, nxt;
-  load(0, cur);
-  for (int z = 0; z < L; ++z) {
-    if (z + 1 < L) load(z + 1, nxt);
+  // DEEP: plane z's loads are issued two planes ahead, three buffers rotating
+  // through the z loop unrolled by 3; otherwise one plane ahead.
+  auto plane = [&](int z, const plane_loads<MB> &cur, plane_loads<MB> &far, int ahead) {
+    if (z + ahead < L) load(z + ahead, far);
     uint64_t g;
     if (COUNTER) {
       if ((z & 1) == 0) blk = counter_block((uint32_t)(c0 + z * L2), p.seed, p.sweep);
@@ -280,7 +282,23 @@ __device__ __forceinline__ void half3d_column(const sweep_args &p, int color, in
       own[m] = down;
     }
     jz_up = cur.jd;
-    cur = nxt;
+  };
+  if constexpr (DEEP) {
+    plane_loads<MB> b0, b1, b2;
+    load(0, b0);
+    if (1 < L) load(1, b1);
+    for (int z = 0; z < L; z += 3) {
+      plane(z, b0, b2, 2);
+      if (z + 1 < L) plane(z + 1, b1, b0, 2);
+      if (z + 2 < L) plane(z + 2, b2, b1, 2);
+    }
+  } else {
+    plane_loads<MB> cur, nxt;
+    load(0, cur);
+    for (int z = 0; z < L; ++z) {
+      plane(z, cur, nxt, 1);
+      cur = nxt;
+    }
   }
 }
 
