@@ -297,8 +297,8 @@ constexpr bool kFusedBatch = false;
 struct tune_cand {
   int fused, block, chunk;
 };
-const tune_cand kCands[4] = {{1, 128, 32}, {1, 256, 32}, {1, 64, 32}, {1, 128, 8}};
-const int kNumCands = 4;
+const tune_cand kCands[2] = {{1, 128, 32}, {1, 64, 32}};
+const int kNumCands = 2;
 
 }  // namespace
 
