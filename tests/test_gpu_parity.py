@@ -355,3 +355,31 @@ def test_geometric_anneal_driver_text_matches_reference(sa, golden):
         txt = sa.anneal_geometric(c["kind"], c["L"], c["T0"], c["alfa"], c["sweeps"], c["nsteps"], c["stride"],
                                   0.0, c["seed"], sa.RNG_REPLAY, precision=9)
         assert txt == c["text"]
+
+
+@pytest.mark.parametrize("mb", [1, 2])
+def test_replay_fused_L128_matches_oracle(sa, orc, mb):
+    """The fused sweep at L=128 (rows are whole waves: the parity-specialised walk)
+    fed the reference's own mt19937_64 stream, against the oracle: two sweeps of two
+    systems, one or two systems per thread."""
+    kind, L, nsys, seed = 2, 128, 2, 777
+    say("replay fused L=128", "mb", mb)
+    lad = sa.Ladder(kind, L, nsys)
+    lad.set_rng(sa.RNG_REPLAY)
+    lad.tune(sa.TUNE_SWEEP3D_KERNEL, 2)
+    lad.tune(sa.TUNE_SYS_PER_THREAD, mb)
+    lad.tune(sa.TUNE_ROWS_PER_TILE, 8)
+    lad.seed_like_main(seed)
+    Ts = [0.7, 1.6]
+    for j in range(nsys):
+        lad.set_temperature(j, Ts[j])
+    jx, jy, jz, s0, g = orc.init_system(kind, L, seed)
+    ref = [s0.copy() for _ in range(nsys)]
+    for _ in range(2):
+        w = g.fill(L ** 3)
+        for j in range(nsys):
+            orc.sweep(kind, L, ref[j], jx, jy, jz, orc.boltz(kind, Ts[j], 0.0), w)
+    lad.sweep(2)
+    for j in range(nsys):
+        assert (lad.spins(j) == ref[j]).all(), j
+    lad.close()
