@@ -64,31 +64,32 @@ def roofline_kernel(fused: bool) -> str:
 def cpu_baseline():
     """The reference's parallel_tempering<cpu_sys> over cpu_3d (compiled from the
     reference sources into oracle/_ref), OpenMP over the temperatures as in the
-    reference, on this host's cores: one step of 2 sweeps of the same ladder."""
+    reference, on this host's cores: one step of 2 sweeps of the same ladder; and
+    one sweep of one system on one core (`one_core`: the reference's sweep is
+    serial)."""
     import oracle
 
-    cores = len(os.sched_getaffinity(0))
-    sweeps = 2
     if oracle.ref_available():
         R = oracle.ref()
+        cores = len(os.sched_getaffinity(0))
+        sweeps = 2
         secs = R.ref_time_pt(KIND, L, NT, np.array(T_LADDER), 1234, sweeps, 1, cores)
-        kind = "reference"
-        sample = f"1 PT step x {sweeps} sweeps of the 32-temperature L=128 ladder (reference cpu_3d, OpenMP)"
-    else:  # the oracle restatement, one system on one core
-        cores = 1
-        sweeps = 1
-        jx, jy, jz, s, g = oracle.init_system(KIND, L, 1234)
-        tab = oracle.boltz(KIND, 1.0)
-        w = g.fill(s.size)
-        t0 = time.perf_counter()
-        oracle.sweep(KIND, L, s, jx, jy, jz, tab, w)
-        secs = time.perf_counter() - t0
-        kind = "port"
-        sample = "1 sweep of one L=128 system (oracle restatement, 1 core)"
-        return dict(value=sweeps * L**3 * 64 / secs / 1e9, unit="attempts/ns", cores=cores, kind=kind,
-                    sample=sample)
-    attempts = NT * sweeps * L**3 * 64
-    return dict(value=attempts / secs / 1e9, unit="attempts/ns", cores=cores, kind=kind, sample=sample)
+        # the reference's sweep is serial: one system on one core as well
+        one = R.ref_time_sweeps(KIND, L, 1.0, 1234, 1)
+        return dict(value=NT * sweeps * L**3 * 64 / secs / 1e9, unit="attempts/ns", cores=cores, kind="reference",
+                    sample=f"1 PT step x {sweeps} sweeps of the 32-temperature L=128 ladder (reference cpu_3d, "
+                           "OpenMP)",
+                    one_core=dict(value=L**3 * 64 / one / 1e9, unit="attempts/ns", cores=1,
+                                  sample="1 sweep of one L=128 system at T=1 (reference cpu_3d)"))
+    # the oracle restatement, one system on one core
+    jx, jy, jz, s, g = oracle.init_system(KIND, L, 1234)
+    tab = oracle.boltz(KIND, 1.0)
+    w = g.fill(s.size)
+    t0 = time.perf_counter()
+    oracle.sweep(KIND, L, s, jx, jy, jz, tab, w)
+    secs = time.perf_counter() - t0
+    return dict(value=L**3 * 64 / secs / 1e9, unit="attempts/ns", cores=1, kind="port",
+                sample="1 sweep of one L=128 system (oracle restatement, 1 core)")
 
 
 def batched_vs_plain(sa, n=512):
