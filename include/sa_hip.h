@@ -33,7 +33,7 @@ typedef struct sa_ladder sa_ladder;
 enum sa_status {
   SA_OK = 0,
   SA_ERR_ARG = -1,          /* invalid argument */
-  SA_ERR_UNSUPPORTED = -2,  /* geometry not supported by the device path (odd L) */
+  SA_ERR_UNSUPPORTED = -2,  /* geometry not supported (L < 3; odd L in a batch or a served request) */
   SA_ERR_HIP = -3,          /* HIP runtime error */
   SA_ERR_NOMEM = -4,        /* device or pinned allocation failed */
   SA_ERR_STATE = -5         /* call not valid in the current state */
@@ -60,7 +60,9 @@ int sa_version(void);
 
 /* Create a ladder of nsys systems (device memory for spins, couplings, tables).
  * Replaces the gpu_2d/gpu_3d constructors (src/gpu_3d.cu:20-77, src/gpu_2d.cu:20-84).
- * The device path needs an even L >= 4 (SA_ERR_UNSUPPORTED otherwise). */
+ * L >= 3 (SA_ERR_UNSUPPORTED otherwise); odd L sweeps each wrap group (a site of the
+ * corner block [0, L-1)^2 with its same-colour partners in the last column / row) in
+ * the reference's visit order. */
 int sa_ladder_create(int kind, int L, int nsys, sa_ladder **out);
 /* Release everything (gpu_free, src/gpu_3d.cu:79-98).  NULL is accepted. */
 int sa_ladder_destroy(sa_ladder *l);

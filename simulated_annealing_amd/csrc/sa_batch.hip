@@ -725,6 +725,8 @@ int sync_batch(sa_batch *b) {
 // tables uploaded on the ladder's own stream.
 int prepare_instance(int kind, int L, int nsys, uint64_t seed, int nsweeps, const double *T, int per_sweep, double h,
                      const pt_state *pt, batch_inst &in) {
+  // the batch kernels run the even-L column/site walks
+  if (L < 4 || (L & 1)) return sa_fail(SA_ERR_UNSUPPORTED, "batch instance: L=%d (batches need an even L >= 4)", L);
   sa_ladder *l = nullptr;
   int rc = sa_ladder_create(kind, L, nsys, &l);
   if (rc) return rc;

@@ -756,6 +756,116 @@ __global__ void __launch_bounds__(256) k_sweep2d_half(sweep_args p, int color) {
   half2d_site<COUNTER, FAST, OPEN, MB>(p, color, k, blockIdx.y * MB);
 }
 
+// ---------------------------------------------------------------- odd L
+// With odd L the wrap neighbours in x and in y have the same colour: in a half-sweep
+// the site (L-1, y) reads the new (0, y), the site (x, L-1) the new (x, 0), and
+// (L-1, L-1) the new (0, L-1) and (L-1, 0) -- each is visited after its partner in
+// the reference's row-major order (src/cpu_3d.cpp:94-105, src/cpu_2dp.cpp:106-114),
+// which sees the partner's new value and the other way round the old one.  These
+// groups (a site of the (L-1) x (L-1) corner block [0, L-1)^2 with its partners in
+// the last column / last row / corner) are closed: every other neighbour of a group
+// site has the other colour and does not change in the half-sweep, and along z a
+// site's same-colour neighbours are its own column.  So one thread walks one group
+// (for 3D along z, plane by plane), updating its sites in visit order and reading
+// every neighbour from memory: its own earlier writes are the new values, all else
+// the old ones.  The corner block holds (L-1)^2/2 sites of each colour.  Open x (2p):
+// the x-wrap has no bond, so the x partner is merely handled by the same thread.
+
+// visit rank of colour-c site (x, y) in plane z (the replay word it consumes): all
+// colour-0 sites come first; a row holds (L-1)/2 sites plus one when it starts at x=0
+__device__ __forceinline__ int64_t odd_rank(int L, int nz, int c, int x, int y, int z) {
+  const int64_t L2 = (int64_t)L * L;
+  const int64_t P0 = (L2 + 1) / 2, Pc = c ? L2 / 2 : P0;  // colour-0 / colour-c sites per plane
+  const int64_t rows_before = (int64_t)y * ((L - 1) / 2) + (c == 0 ? (y + 1) / 2 : y / 2);
+  return (c ? nz * P0 : 0) + z * Pc + rows_before + (x - ((y + c) & 1)) / 2;
+}
+
+template <bool COUNTER, bool FAST>
+__device__ __forceinline__ void odd3d_site(const sweep_args &p, uint64_t *s, const sys_table &t, int color, int x,
+                                           int y, int z) {
+  const int L = p.L;
+  const int64_t L2 = (int64_t)L * L;
+  const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
+  const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
+  const int zm = z == 0 ? L - 1 : z - 1, zp = z == L - 1 ? 0 : z + 1;
+  const int64_t row = (int64_t)L * y + L2 * z;
+  const int64_t i = x + row;
+  const int64_t iw = xm + row, ie = xp + row;
+  const int64_t in = x + (int64_t)L * ym + L2 * z, is = x + (int64_t)L * yp + L2 * z;
+  const int64_t iu = x + (int64_t)L * y + L2 * zm, id = x + (int64_t)L * y + L2 * zp;
+  const uint64_t si = s[i];
+  const count3 c = count6(xor3(p.jx[i], si, s[iw]), xor3(p.jx[ie], si, s[ie]), xor3(p.jy[i], si, s[in]),
+                          xor3(p.jy[is], si, s[is]), xor3(p.jz[i], si, s[iu]), xor3(p.jz[id], si, s[id]));
+  uint64_t g;
+  if (COUNTER) {
+    const u32x4 r = counter_block((uint32_t)(x + L * y + L2 * (z & ~1)), p.seed, p.sweep);
+    g = counter_half(r, z & 1);
+  } else {
+    g = p.words[odd_rank(L, L, color, x, y, z)];
+  }
+  s[i] = FAST ? update_fast3(t, g, c, si) : update_general(t, 0, g, c, si, 7);
+}
+
+template <bool COUNTER, bool FAST>
+__global__ void __launch_bounds__(256) k_sweep3d_odd(sweep_args p, int color) {
+  const int L = p.L, h = (L - 1) >> 1;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (L - 1) * h) return;
+  const int y = k / h, x = 2 * (k - y * h) + ((y + color) & 1);
+  const int sys = blockIdx.y;
+  uint64_t *s = p.s + (int64_t)sys * p.N;
+  const sys_table &t = p.tab[sys];
+  for (int z = 0; z < L; ++z) {
+    odd3d_site<COUNTER, FAST>(p, s, t, color, x, y, z);
+    if (x == 0) odd3d_site<COUNTER, FAST>(p, s, t, color, L - 1, y, z);  // (L-1, y) reads the new (0, y)
+    if (y == 0) odd3d_site<COUNTER, FAST>(p, s, t, color, x, L - 1, z);  // (x, L-1) reads the new (x, 0)
+    if (x == 0 && y == 0) odd3d_site<COUNTER, FAST>(p, s, t, color, L - 1, L - 1, z);
+  }
+}
+
+template <bool COUNTER, bool FAST, bool OPEN>
+__device__ __forceinline__ void odd2d_site(const sweep_args &p, uint64_t *s, const sys_table &t, int color, int x,
+                                           int y) {
+  const int L = p.L;
+  const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
+  const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
+  const int64_t i = x + (int64_t)L * y;
+  const int64_t iw = xm + (int64_t)L * y, ie = xp + (int64_t)L * y;
+  const int64_t in = x + (int64_t)L * ym, is = x + (int64_t)L * yp;
+  const uint64_t si = s[i];
+  uint64_t bw = xor3(p.jx[i], si, s[iw]);
+  uint64_t be = xor3(p.jx[ie], si, s[ie]);
+  if (OPEN) {  // missing open-x bond counts as satisfied (src/cpu_2dp.cpp:338-339)
+    if (x == 0) bw = ~0ULL;
+    if (x == L - 1) be = ~0ULL;
+  }
+  const count3 c = count4(bw, be, xor3(p.jy[i], si, s[in]), xor3(p.jy[is], si, s[is]));
+  uint64_t g;
+  if (COUNTER) {
+    const u32x4 r = counter_block((uint32_t)(((x & ~3) | (x & 1)) + L * y), p.seed, p.sweep);
+    g = counter_half(r, (x >> 1) & 1);
+  } else {
+    g = p.words[odd_rank(L, 1, color, x, y, 0)];
+  }
+  const bool border = OPEN && (x == 0 || x == L - 1);
+  s[i] = (FAST && !OPEN) ? update_fast2(t, g, c, si) : update_general(t, border ? 2 : 0, g, c, si, 5);
+}
+
+template <bool COUNTER, bool FAST, bool OPEN>
+__global__ void __launch_bounds__(256) k_sweep2d_odd(sweep_args p, int color) {
+  const int L = p.L, h = (L - 1) >> 1;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (L - 1) * h) return;
+  const int y = k / h, x = 2 * (k - y * h) + ((y + color) & 1);
+  const int sys = blockIdx.y;
+  uint64_t *s = p.s + (int64_t)sys * p.N;
+  const sys_table &t = p.tab[sys];
+  odd2d_site<COUNTER, FAST, OPEN>(p, s, t, color, x, y);
+  if (x == 0) odd2d_site<COUNTER, FAST, OPEN>(p, s, t, color, L - 1, y);
+  if (y == 0) odd2d_site<COUNTER, FAST, OPEN>(p, s, t, color, x, L - 1);
+  if (x == 0 && y == 0) odd2d_site<COUNTER, FAST, OPEN>(p, s, t, color, L - 1, L - 1);
+}
+
 // ---------------------------------------------------------------- measure
 // Per system and lane: number of satisfied -x,-y[,-z] bonds and of set spins.
 // Each lane keeps bit-sliced counters (bit k of plane b = bit b of lane k's count):

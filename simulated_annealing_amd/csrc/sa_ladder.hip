@@ -88,6 +88,17 @@ int pick_mb(const sa_ladder *l) {
 
 template <bool COUNTER, bool FAST>
 void launch_half(const sa_ladder *l, const sa::sweep_args &a, int color) {
+  if (l->L & 1) {  // odd L: one thread per wrap group of the corner block, one system per grid row
+    const int h = (l->L - 1) / 2;
+    const dim3 grid((unsigned)(((l->L - 1) * h + 255) / 256), (unsigned)l->nsys);
+    if (l->kind == SA_KIND_3D)
+      hipLaunchKernelGGL((sa::k_sweep3d_odd<COUNTER, FAST>), grid, dim3(256), 0, l->stream, a, color);
+    else if (l->kind == SA_KIND_2D)
+      hipLaunchKernelGGL((sa::k_sweep2d_odd<COUNTER, FAST, false>), grid, dim3(256), 0, l->stream, a, color);
+    else
+      hipLaunchKernelGGL((sa::k_sweep2d_odd<COUNTER, false, true>), grid, dim3(256), 0, l->stream, a, color);
+    return;
+  }
   const int mb = pick_mb(l);
   if (l->kind == SA_KIND_3D) {
     if (mb == 4) launch3d<COUNTER, FAST, 4>(a, color, l->stream);
@@ -248,7 +259,7 @@ int sa_ladder_create(int kind, int L, int nsys, sa_ladder **out) {
   *out = nullptr;
   if (kind < SA_KIND_2P || kind > SA_KIND_3D) return sa_fail(SA_ERR_ARG, "unknown kind %d", kind);
   if (nsys < 1) return sa_fail(SA_ERR_ARG, "nsys must be >= 1");
-  if (L < 4 || (L & 1)) return sa_fail(SA_ERR_UNSUPPORTED, "device path needs an even L >= 4 (got %d)", L);
+  if (L < 3) return sa_fail(SA_ERR_UNSUPPORTED, "device path needs L >= 3 (got %d)", L);
   if ((kind == SA_KIND_3D && L > 1024) || (kind != SA_KIND_3D && L > 32768))
     return sa_fail(SA_ERR_ARG, "L=%d too large", L);
   sa_ladder *l = new sa_ladder();

@@ -233,9 +233,13 @@ def test_ferromagnetic_ground_state_L128(sa):
 
 def test_unsupported_geometry_is_an_error(sa):
     with pytest.raises(sa.SAError):
-        sa.Ladder(2, 7, 1)
+        sa.Ladder(2, 2, 1)
     with pytest.raises(sa.SAError):
         sa.Ladder(1, 2, 1)
+    b = sa.Batch()
+    with pytest.raises(sa.SAError):
+        b.add(2, 9, 1, 3, T=np.ones(1))  # batches run even L only
+    b.close()
 
 
 # ---- every 3D kernel variant against the oracle ----------------------------------------
@@ -383,3 +387,46 @@ def test_replay_fused_L128_matches_oracle(sa, orc, mb):
     for j in range(nsys):
         assert (lad.spins(j) == ref[j]).all(), j
     lad.close()
+
+
+# ---- odd L: the wrap neighbours share a colour --------------------------------------------
+@pytest.mark.parametrize("kind,L,nsys", [(2, 3, 1), (2, 5, 2), (2, 7, 1), (2, 9, 3), (2, 15, 1), (1, 3, 1),
+                                         (1, 5, 1), (1, 9, 2), (1, 21, 1), (0, 5, 1), (0, 7, 2), (0, 13, 1)])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_odd_L_matches_oracle(sa, orc, kind, L, nsys, mode):
+    """Odd L (one thread per wrap group, sites in the reference's visit order) against
+    the oracle, which the compiled reference pins for odd L as well: three sweeps, per
+    system its own temperature, a field on the even systems of the counter runs."""
+    say("odd L", "kind", kind, "L", L, "nsys", nsys, "mode", mode)
+    seed, nsw = 101 + L + kind, 3
+    lad = sa.Ladder(kind, L, nsys)
+    lad.set_rng(mode)
+    lad.seed_like_main(seed)
+    Ts = [0.5 + 0.6 * j for j in range(nsys)]
+    hs = [0.3 if (mode == 1 and j % 2 == 0) else 0.0 for j in range(nsys)]
+    for j in range(nsys):
+        lad.set_temperature(j, Ts[j], hs[j])
+    jx, jy, jz, s0, g = orc.init_system(kind, L, seed)
+    N = s0.size
+    ref = [s0.copy() for _ in range(nsys)]
+    for t in range(nsw):
+        w = orc.counter_words(kind, L, seed, t) if mode == 1 else g.fill(N)
+        for j in range(nsys):
+            orc.sweep(kind, L, ref[j], jx, jy, jz, orc.boltz(kind, Ts[j], hs[j]), w)
+    lad.sweep(nsw)
+    for j in range(nsys):
+        assert (lad.spins(j) == ref[j]).all(), j
+    E, M = lad.measure()
+    for j in range(nsys):
+        e, m = orc.measure(kind, L, ref[j], jx, jy, jz)
+        assert (E[j] == e).all() and (M[j] == m).all()
+    lad.close()
+
+
+def test_odd_L_pt_matches_oracle(sa, orc):
+    """Parallel tempering on an odd lattice (replay stream) against the oracle's PT."""
+    kind, L, T = 2, 9, np.array([0.5, 0.9, 1.6, 3.0])
+    got = sa.pt_run(kind, L, T, 0.0, 55, 4, 6, sa.RNG_REPLAY)
+    want = orc.pt_run(kind, L, T, 0.0, 55, 4, 6, rng_mode=0)
+    for k in ("pairs", "us", "masks", "E", "spins"):
+        assert (got[k] == want[k]).all(), k
