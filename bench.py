@@ -143,7 +143,7 @@ def serving(sa, n=1000, rate=40.0, budget=0, max_active=0):
     """The serving form: a Poisson stream of anneal / parallel-tempering requests.
 
     Requests (simulated_annealing_amd.workloads.serving_requests): 2D L log-uniform
-    in [16, 256] or 3D L log-uniform in [8, 64] (even L), sweeps log-uniform in
+    in [16, 256] or 3D L log-uniform in [8, 64] (odd L included), sweeps log-uniform in
     [100, 10000], half of them PT with 8..32 temperatures geometric in [0.5, 2.0]
     exchanging every 10 sweeps, the others a linear anneal 3.0 -> 0.1; per-request
     seeds.  All requests are submitted up front with their arrival times; the

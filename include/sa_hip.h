@@ -33,7 +33,7 @@ typedef struct sa_ladder sa_ladder;
 enum sa_status {
   SA_OK = 0,
   SA_ERR_ARG = -1,          /* invalid argument */
-  SA_ERR_UNSUPPORTED = -2,  /* geometry not supported (L < 3; odd L in a batch or a served request) */
+  SA_ERR_UNSUPPORTED = -2,  /* geometry not supported (L < 3) */
   SA_ERR_HIP = -3,          /* HIP runtime error */
   SA_ERR_NOMEM = -4,        /* device or pinned allocation failed */
   SA_ERR_STATE = -5         /* call not valid in the current state */

@@ -57,9 +57,23 @@ __device__ __forceinline__ void batch_item(const inst_desc &in, int t, int color
   if (in.cur) a.s = in.s2;  // half-sweeps update the current array in place
   if (in.per_sweep) a.tab += (int64_t)t * a.nsys;
   a.sweep = (uint64_t)t;
+  if (a.L & 1) {  // odd L: k is a member lane (3D) or a wrap group (2D), sa_kernels.hpp
+    if (in.kind == SA_KIND_3D) sa::odd3d_lane<true, FAST>(a, color, k, sys);
+    else if (in.kind == SA_KIND_2D) sa::odd2d_group<true, FAST, false>(a, color, k, sys);
+    else sa::odd2d_group<true, false, true>(a, color, k, sys);
+    return;
+  }
   if (in.kind == SA_KIND_3D) sa::half3d_column<true, FAST, 1, true>(a, color, k, sys);
   else if (in.kind == SA_KIND_2D) sa::half2d_site<true, FAST, false, 1>(a, color, k, sys);
   else sa::half2d_site<true, false, true, 1>(a, color, k, sys);
+}
+
+// work items of one system in a half-sweep: columns (3D) or sites (2D) of one
+// colour for even L; for odd L the member lanes (3D, a multiple of 64 so that each
+// system starts at a wave boundary) or the wrap groups (2D)
+__host__ __device__ __forceinline__ int half_items(int kind, int L) {
+  if (!(L & 1)) return L * (L >> 1);
+  return kind == SA_KIND_3D ? sa::odd_width(L) : sa::odd_groups(L);
 }
 
 // ctrl[0] = blocks in the block map, ctrl[1] = descriptors; s = the sweep's offset in
@@ -72,7 +86,7 @@ __global__ void __launch_bounds__(256) k_batch_half(const inst_desc *__restrict_
     const inst_desc &in = d[bm.x];
     const int t = in.t + s;
     if (t >= in.nsweeps) continue;
-    const int L = in.a.L, work = L * (L >> 1);
+    const int work = half_items(in.kind, in.a.L);
     const int item = bm.y + (int)threadIdx.x;
     if (item >= work * in.a.nsys) continue;
     const int sys = item / work, k = item - sys * work;
@@ -725,8 +739,6 @@ int sync_batch(sa_batch *b) {
 // tables uploaded on the ladder's own stream.
 int prepare_instance(int kind, int L, int nsys, uint64_t seed, int nsweeps, const double *T, int per_sweep, double h,
                      const pt_state *pt, batch_inst &in) {
-  // the batch kernels run the even-L column/site walks
-  if (L < 4 || (L & 1)) return sa_fail(SA_ERR_UNSUPPORTED, "batch instance: L=%d (batches need an even L >= 4)", L);
   sa_ladder *l = nullptr;
   int rc = sa_ladder_create(kind, L, nsys, &l);
   if (rc) return rc;
@@ -755,7 +767,7 @@ int prepare_instance(int kind, int L, int nsys, uint64_t seed, int nsweeps, cons
   in.nsweeps = nsweeps;
   in.per_sweep = per_sweep ? 1 : 0;
   in.fast = fast;
-  in.work = (int64_t)L * (L / 2) * nsys;
+  in.work = (int64_t)half_items(kind, L) * nsys;
   if (!rc && hipMalloc(&in.tabs, sizeof(sa::sys_table) * ntab) != hipSuccess) rc = sa_fail(SA_ERR_NOMEM, "tables");
   // on the ladder's stream (waited for below): admission may run while a chunk is
   // on the batch stream, and a null-stream copy would wait for it

@@ -765,11 +765,11 @@ __global__ void __launch_bounds__(256) k_sweep2d_half(sweep_args p, int color) {
 // groups (a site of the (L-1) x (L-1) corner block [0, L-1)^2 with its partners in
 // the last column / last row / corner) are closed: every other neighbour of a group
 // site has the other colour and does not change in the half-sweep, and along z a
-// site's same-colour neighbours are its own column.  So one thread walks one group
-// (for 3D along z, plane by plane), updating its sites in visit order and reading
-// every neighbour from memory: its own earlier writes are the new values, all else
-// the old ones.  The corner block holds (L-1)^2/2 sites of each colour.  Open x (2p):
-// the x-wrap has no bond, so the x partner is merely handled by the same thread.
+// site's same-colour neighbours are its own column.  A group's sites are updated
+// in visit order with the new words of its own sites passed on in registers (the
+// partners' +x / +y neighbours); every other neighbour is an old word.  The corner
+// block holds (L-1)^2/2 sites of each colour.  Open x (2p): the x-wrap has no bond,
+// so the x partner is merely handled in the same group.
 
 // visit rank of colour-c site (x, y) in plane z (the replay word it consumes): all
 // colour-0 sites come first; a row holds (L-1)/2 sites plus one when it starts at x=0
@@ -780,90 +780,224 @@ __device__ __forceinline__ int64_t odd_rank(int L, int nz, int c, int x, int y, 
   return (c ? nz * P0 : 0) + z * Pc + rows_before + (x - ((y + c) & 1)) / 2;
 }
 
-template <bool COUNTER, bool FAST>
-__device__ __forceinline__ void odd3d_site(const sweep_args &p, uint64_t *s, const sys_table &t, int color, int x,
-                                           int y, int z) {
-  const int L = p.L;
-  const int64_t L2 = (int64_t)L * L;
+// groups of one colour: (L-1)^2/2 corner-block sites, one per thread
+__host__ __device__ __forceinline__ int odd_groups(int L) { return (L - 1) * ((L - 1) >> 1); }
+
+// Group k's members: 0 = (x, y) in the corner block, 1 = (L-1, y) when x = 0, 2 = (x, L-1)
+// when y = 0, 3 = (L-1, L-1) when x = y = 0.  Forwarded new words: member 1's +x
+// neighbour is member 0, member 2's +y neighbour is member 0, member 3's are members 2
+// (+x) and 1 (+y).
+struct odd_member {
+  int c0, cw, ce, cn, cs;  // in-plane offsets of the site and its -x, +x, -y, +y neighbours
+};
+__device__ __forceinline__ odd_member odd_offsets(int L, int x, int y) {
   const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
   const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
-  const int zm = z == 0 ? L - 1 : z - 1, zp = z == L - 1 ? 0 : z + 1;
-  const int64_t row = (int64_t)L * y + L2 * z;
-  const int64_t i = x + row;
-  const int64_t iw = xm + row, ie = xp + row;
-  const int64_t in = x + (int64_t)L * ym + L2 * z, is = x + (int64_t)L * yp + L2 * z;
-  const int64_t iu = x + (int64_t)L * y + L2 * zm, id = x + (int64_t)L * y + L2 * zp;
-  const uint64_t si = s[i];
-  const count3 c = count6(xor3(p.jx[i], si, s[iw]), xor3(p.jx[ie], si, s[ie]), xor3(p.jy[i], si, s[in]),
-                          xor3(p.jy[is], si, s[is]), xor3(p.jz[i], si, s[iu]), xor3(p.jz[id], si, s[id]));
-  uint64_t g;
-  if (COUNTER) {
-    const u32x4 r = counter_block((uint32_t)(x + L * y + L2 * (z & ~1)), p.seed, p.sweep);
-    g = counter_half(r, z & 1);
+  return odd_member{x + L * y, xm + L * y, xp + L * y, x + L * ym, x + L * yp};
+}
+struct odd_loads {
+  uint64_t jw, je, jn, js, jd;  // couplings: -x, +x, -y, +y, +z bonds
+  uint64_t w, e, n, s, d;       // old neighbour words (e / s not loaded where forwarded)
+};
+
+// 3D: one lane per member.  Lane i of a system's odd_width(L) lanes (systems start
+// at a wave boundary):
+//   i < 4h (h = (L-1)/2): lane pairs (2q, 2q+1) = (corner-block site, its partner)
+//     colour 0: q = 0: (0,0) & (0,L-1);  q = 1: (L-1,0) & (L-1,L-1);
+//               q = 2..h: (2(q-1),0) & (2(q-1),L-1);  q = h+1..2h-1: (0,y) & (L-1,y), y = 2(q-h)
+//     colour 1: q < h: (1+2q,0) & (1+2q,L-1);  q = h..2h-1: (0,y) & (L-1,y), y = 1+2(q-h)
+//   then the other corner-block sites (1 <= x, y <= L-2), two rows at a time;
+//   lanes from the colour's site count on are idle.
+// Member 0 sites walk z in step tau = z, members 1 and 2 in tau = z+1, member 3 in
+// tau = z+2: a partner updates plane z one step after the site it reads the new
+// word of, which then sits in that lane's `up` register and is passed by a wave
+// shuffle.  The other same-colour wrap neighbours (a member 0 site reads its
+// partners, members 1 and 2 read (L-1,L-1)) are updated one step or more after the
+// reader has used their old word, and all other neighbours have the other colour,
+// so every lane prefetches its plane loads two steps ahead like the even walk.
+__host__ __device__ __forceinline__ int odd_width(int L) { return ((L * L + 1) / 2 + 63) & ~63; }
+
+template <bool COUNTER, bool FAST>
+__device__ __forceinline__ void odd3d_lane(const sweep_args &p, int color, int i, int sys) {
+  const int L = p.L, h = (L - 1) >> 1;
+  const int64_t L2 = (int64_t)L * L;
+  const int64_t Pc = color ? L2 / 2 : (L2 + 1) / 2;  // sites of this colour per plane
+  const int lane = i & 63;
+  const bool live = i < Pc;
+  int x = 0, y = 0, skew = 0, srcE = lane, srcS = lane;  // lanes of the forwarded +x / +y words
+  if (i < 4 * h) {
+    const int q = i >> 1, r = i & 1;
+    int xr = -1, yr = -1;  // row-0 pair at x = xr, or x = 0 pair at row yr
+    if (color == 0) {
+      if (q == 1) {  // (L-1,0) reads the new (0,0); (L-1,L-1) reads (0,L-1) and (L-1,0)
+        x = L - 1;
+        y = r ? L - 1 : 0;
+        skew = 1 + r;
+        srcE = lane - 2;
+        if (r) srcS = lane - 1;
+      } else if (q <= h) {
+        xr = q == 0 ? 0 : 2 * (q - 1);
+      } else {
+        yr = 2 * (q - h);
+      }
+    } else {
+      if (q < h) xr = 1 + 2 * q;
+      else yr = 1 + 2 * (q - h);
+    }
+    if (xr >= 0) {  // (xr, 0) and (xr, L-1), which reads the new (xr, 0) as its +y neighbour
+      x = xr;
+      y = r ? L - 1 : 0;
+      skew = r;
+      if (r) srcS = lane - 1;
+    } else if (yr >= 0) {  // (0, yr) and (L-1, yr), which reads the new (0, yr) as its +x neighbour
+      x = r ? L - 1 : 0;
+      y = yr;
+      skew = r;
+      if (r) srcE = lane - 1;
+    }
   } else {
-    g = p.words[odd_rank(L, L, color, x, y, z)];
+    // rows 1 .. L-2 in pairs; a row whose x = 0 site has this colour holds h-1
+    // sites (x = 2, 4, ..), the other h (x = 1, 3, ..); row 1 is of the second kind
+    // for colour 0 and of the first for colour 1
+    const int sidx = i - 4 * h, r2 = sidx / (2 * h - 1), rem = sidx - r2 * (2 * h - 1);
+    const int y0 = 1 + 2 * r2;
+    const int n0 = color ? h - 1 : h;  // sites in row y0
+    if (rem < n0) {
+      y = y0;
+      x = (color ? 2 : 1) + 2 * rem;
+    } else {
+      y = y0 + 1;
+      x = (color ? 1 : 2) + 2 * (rem - n0);
+    }
   }
-  s[i] = FAST ? update_fast3(t, g, c, si) : update_general(t, 0, g, c, si, 7);
+  const odd_member o = odd_offsets(L, x, y);
+  const bool fwdE = srcE != lane, fwdS = srcS != lane;
+  uint64_t *s = p.s + (int64_t)sys * p.N;
+  const sys_table &t = p.tab[sys];
+  const int64_t rk = live ? odd_rank(L, L, color, x, y, 0) : 0;
+  uint64_t own = 0, up = 0, first = 0, jzu = 0;
+  if (live) {
+    own = s[o.c0];
+    up = s[o.c0 + (L - 1) * L2];  // old plane L-1
+    jzu = p.jz[o.c0];
+  }
+  auto load = [&](int z, odd_loads &q) {
+    if (!live || z < 0 || z >= L) return;
+    const int64_t P = z * L2, Pd = z == L - 1 ? 0 : P + L2;
+    q.jw = p.jx[P + o.c0];
+    q.je = p.jx[P + o.ce];
+    q.jn = p.jy[P + o.c0];
+    q.js = p.jy[P + o.cs];
+    q.jd = p.jz[Pd + o.c0];
+    q.w = s[P + o.cw];
+    q.n = s[P + o.cn];
+    q.e = fwdE ? 0 : s[P + o.ce];
+    q.s = fwdS ? 0 : s[P + o.cs];
+    q.d = z == L - 1 ? 0 : s[Pd + o.c0];
+  };
+  u32x4 blk{0, 0, 0, 0};
+  // step tau: the forwarded words are the source lanes' `up` (their new word of the
+  // plane this lane updates now); plane tau-skew uses the buffer filled at tau-2
+  auto step = [&](int tau, const odd_loads &cur, odd_loads &far) {
+    const uint64_t fe = __shfl(up, srcE), fs = __shfl(up, srcS);
+    load(tau + 2 - skew, far);
+    const int z = tau - skew;
+    if (!live || z < 0 || z >= L) return;
+    const uint64_t si = own;
+    const uint64_t down = z == L - 1 ? first : cur.d;
+    const count3 c = count6(xor3(cur.jw, si, cur.w), xor3(cur.je, si, fwdE ? fe : cur.e), xor3(cur.jn, si, cur.n),
+                            xor3(cur.js, si, fwdS ? fs : cur.s), xor3(jzu, si, up), xor3(cur.jd, si, down));
+    uint64_t g;
+    if (COUNTER) {
+      if ((z & 1) == 0) blk = counter_block((uint32_t)(o.c0 + L2 * z), p.seed, p.sweep);
+      g = counter_half(blk, z & 1);
+    } else {
+      g = p.words[rk + z * Pc];
+    }
+    const uint64_t sn = FAST ? update_fast3(t, g, c, si) : update_general(t, 0, g, c, si, 7);
+    s[z * L2 + o.c0] = sn;
+    up = sn;
+    if (z == 0) first = sn;
+    own = down;
+    jzu = cur.jd;
+  };
+  odd_loads b0{}, b1{}, b2{};
+  load(0 - skew, b0);
+  load(1 - skew, b1);
+  const int steps = L + 2;
+  for (int tau = 0; tau < steps; tau += 3) {
+    step(tau, b0, b2);
+    if (tau + 1 < steps) step(tau + 1, b1, b0);
+    if (tau + 2 < steps) step(tau + 2, b2, b1);
+  }
 }
 
 template <bool COUNTER, bool FAST>
 __global__ void __launch_bounds__(256) k_sweep3d_odd(sweep_args p, int color) {
-  const int L = p.L, h = (L - 1) >> 1;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= (L - 1) * h) return;
-  const int y = k / h, x = 2 * (k - y * h) + ((y + color) & 1);
-  const int sys = blockIdx.y;
-  uint64_t *s = p.s + (int64_t)sys * p.N;
-  const sys_table &t = p.tab[sys];
-  for (int z = 0; z < L; ++z) {
-    odd3d_site<COUNTER, FAST>(p, s, t, color, x, y, z);
-    if (x == 0) odd3d_site<COUNTER, FAST>(p, s, t, color, L - 1, y, z);  // (L-1, y) reads the new (0, y)
-    if (y == 0) odd3d_site<COUNTER, FAST>(p, s, t, color, x, L - 1, z);  // (x, L-1) reads the new (x, 0)
-    if (x == 0 && y == 0) odd3d_site<COUNTER, FAST>(p, s, t, color, L - 1, L - 1, z);
-  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < odd_width(p.L)) odd3d_lane<COUNTER, FAST>(p, color, i, blockIdx.y);  // whole waves
 }
 
+// 2D: all members' words are loaded first, then updated in visit order
 template <bool COUNTER, bool FAST, bool OPEN>
-__device__ __forceinline__ void odd2d_site(const sweep_args &p, uint64_t *s, const sys_table &t, int color, int x,
-                                           int y) {
-  const int L = p.L;
-  const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
-  const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
-  const int64_t i = x + (int64_t)L * y;
-  const int64_t iw = xm + (int64_t)L * y, ie = xp + (int64_t)L * y;
-  const int64_t in = x + (int64_t)L * ym, is = x + (int64_t)L * yp;
-  const uint64_t si = s[i];
-  uint64_t bw = xor3(p.jx[i], si, s[iw]);
-  uint64_t be = xor3(p.jx[ie], si, s[ie]);
-  if (OPEN) {  // missing open-x bond counts as satisfied (src/cpu_2dp.cpp:338-339)
-    if (x == 0) bw = ~0ULL;
-    if (x == L - 1) be = ~0ULL;
+__device__ __forceinline__ void odd2d_group(const sweep_args &p, int color, int k, int sys) {
+  const int L = p.L, h = (L - 1) >> 1;
+  const int y = k / h, x = 2 * (k - y * h) + ((y + color) & 1);
+  uint64_t *s = p.s + (int64_t)sys * p.N;
+  const sys_table &t = p.tab[sys];
+  const bool pres[4] = {true, x == 0, y == 0, x == 0 && y == 0};
+  const int mx[4] = {x, L - 1, x, L - 1}, my[4] = {y, y, L - 1, L - 1};
+  odd_member o[4];
+  odd_loads b[4];
+  uint64_t own[4], nw[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    nw[j] = 0;
+    if (!pres[j]) continue;
+    o[j] = odd_offsets(L, mx[j], my[j]);
+    odd_loads &q = b[j];
+    q.jw = p.jx[o[j].c0];
+    q.je = p.jx[o[j].ce];
+    q.jn = p.jy[o[j].c0];
+    q.js = p.jy[o[j].cs];
+    q.w = s[o[j].cw];
+    q.n = s[o[j].cn];
+    q.e = (j == 1 || j == 3) ? 0 : s[o[j].ce];
+    q.s = (j >= 2) ? 0 : s[o[j].cs];
+    own[j] = s[o[j].c0];
   }
-  const count3 c = count4(bw, be, xor3(p.jy[i], si, s[in]), xor3(p.jy[is], si, s[is]));
-  uint64_t g;
-  if (COUNTER) {
-    const u32x4 r = counter_block((uint32_t)(((x & ~3) | (x & 1)) + L * y), p.seed, p.sweep);
-    g = counter_half(r, (x >> 1) & 1);
-  } else {
-    g = p.words[odd_rank(L, 1, color, x, y, 0)];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (!pres[j]) continue;
+    const odd_loads &q = b[j];
+    const int xj = mx[j], yj = my[j];
+    const uint64_t si = own[j];
+    const uint64_t e = j == 1 ? nw[0] : (j == 3 ? nw[2] : q.e);
+    const uint64_t sv = j == 2 ? nw[0] : (j == 3 ? nw[1] : q.s);
+    uint64_t bw = xor3(q.jw, si, q.w), be = xor3(q.je, si, e);
+    if (OPEN) {  // missing open-x bond counts as satisfied (src/cpu_2dp.cpp:338-339)
+      if (xj == 0) bw = ~0ULL;
+      if (xj == L - 1) be = ~0ULL;
+    }
+    const count3 c = count4(bw, be, xor3(q.jn, si, q.n), xor3(q.js, si, sv));
+    uint64_t g;
+    if (COUNTER) {
+      const u32x4 r = counter_block((uint32_t)(((xj & ~3) | (xj & 1)) + L * yj), p.seed, p.sweep);
+      g = counter_half(r, (xj >> 1) & 1);
+    } else {
+      g = p.words[odd_rank(L, 1, color, xj, yj, 0)];
+    }
+    const bool border = OPEN && (xj == 0 || xj == L - 1);
+    const uint64_t sn = (FAST && !OPEN) ? update_fast2(t, g, c, si) : update_general(t, border ? 2 : 0, g, c, si, 5);
+    s[o[j].c0] = sn;
+    nw[j] = sn;
   }
-  const bool border = OPEN && (x == 0 || x == L - 1);
-  s[i] = (FAST && !OPEN) ? update_fast2(t, g, c, si) : update_general(t, border ? 2 : 0, g, c, si, 5);
 }
 
 template <bool COUNTER, bool FAST, bool OPEN>
 __global__ void __launch_bounds__(256) k_sweep2d_odd(sweep_args p, int color) {
-  const int L = p.L, h = (L - 1) >> 1;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= (L - 1) * h) return;
-  const int y = k / h, x = 2 * (k - y * h) + ((y + color) & 1);
-  const int sys = blockIdx.y;
-  uint64_t *s = p.s + (int64_t)sys * p.N;
-  const sys_table &t = p.tab[sys];
-  odd2d_site<COUNTER, FAST, OPEN>(p, s, t, color, x, y);
-  if (x == 0) odd2d_site<COUNTER, FAST, OPEN>(p, s, t, color, L - 1, y);
-  if (y == 0) odd2d_site<COUNTER, FAST, OPEN>(p, s, t, color, x, L - 1);
-  if (x == 0 && y == 0) odd2d_site<COUNTER, FAST, OPEN>(p, s, t, color, L - 1, L - 1);
+  if (k < odd_groups(p.L)) odd2d_group<COUNTER, FAST, OPEN>(p, color, k, blockIdx.y);
 }
 
 // ---------------------------------------------------------------- measure

@@ -88,9 +88,9 @@ int pick_mb(const sa_ladder *l) {
 
 template <bool COUNTER, bool FAST>
 void launch_half(const sa_ladder *l, const sa::sweep_args &a, int color) {
-  if (l->L & 1) {  // odd L: one thread per wrap group of the corner block, one system per grid row
-    const int h = (l->L - 1) / 2;
-    const dim3 grid((unsigned)(((l->L - 1) * h + 255) / 256), (unsigned)l->nsys);
+  if (l->L & 1) {  // odd L: 3D one lane per member, 2D one thread per wrap group; one system per grid row
+    const int items = l->kind == SA_KIND_3D ? sa::odd_width(l->L) : sa::odd_groups(l->L);
+    const dim3 grid((unsigned)((items + 255) / 256), (unsigned)l->nsys);
     if (l->kind == SA_KIND_3D)
       hipLaunchKernelGGL((sa::k_sweep3d_odd<COUNTER, FAST>), grid, dim3(256), 0, l->stream, a, color);
     else if (l->kind == SA_KIND_2D)

@@ -57,7 +57,7 @@ double request_attempts(const sa_request &r) {
 
 int check_request(const sa_request &r) {
   if (r.kind < SA_KIND_2P || r.kind > SA_KIND_3D) return sa_fail(SA_ERR_ARG, "request: unknown kind %d", r.kind);
-  if (r.L < 4 || (r.L & 1)) return sa_fail(SA_ERR_UNSUPPORTED, "request: L=%d (the device sweep needs even L >= 4)", r.L);
+  if (r.L < 3) return sa_fail(SA_ERR_UNSUPPORTED, "request: L=%d (the device sweep needs L >= 3)", r.L);
   if (r.mode == SA_REQ_ANNEAL) {
     if (r.nsweeps < 1) return sa_fail(SA_ERR_ARG, "request: anneal needs nsweeps >= 1");
   } else if (r.mode == SA_REQ_PT) {

@@ -6,7 +6,7 @@ linear anneal 3.0 -> 0.1 over a sweep count from {500, 1000, 2000}.
 
 serving_requests: a Poisson stream of anneal / parallel-tempering requests, 2D
 with L log-uniform in [16, 256] or 3D with L log-uniform in [8, 64] (rounded to
-the nearest even L, which the device sweep needs), sweeps log-uniform in
+the nearest integer, odd L included), sweeps log-uniform in
 [100, 10000]; a PT request has 8..32 temperatures geometric in [0.5, 2.0] and
 exchanges every 10 sweeps.
 """
@@ -79,8 +79,8 @@ class Request:
         return self.sites * 64 * self.nT * self.nsweeps
 
 
-def _even(x: float, lo: int, hi: int) -> int:
-    return int(min(hi, max(lo, 2 * round(x / 2))))
+def _log_uniform_L(rng, lo: int, hi: int) -> int:
+    return int(min(hi, max(lo, round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))))
 
 
 def serving_requests(n: int = 20000, rate: float = 50.0, seed: int = 7, p3d: float = 0.5, p_pt: float = 0.5,
@@ -92,9 +92,9 @@ def serving_requests(n: int = 20000, rate: float = 50.0, seed: int = 7, p3d: flo
     for i in range(n):
         t += rng.exponential(1.0 / rate)
         if rng.random() < p3d:
-            kind, L = KIND_3D, _even(np.exp(rng.uniform(np.log(8), np.log(64))), 8, 64)
+            kind, L = KIND_3D, _log_uniform_L(rng, 8, 64)
         else:
-            kind, L = KIND_2D, _even(np.exp(rng.uniform(np.log(16), np.log(256))), 16, 256)
+            kind, L = KIND_2D, _log_uniform_L(rng, 16, 256)
         nsw = int(round(np.exp(rng.uniform(np.log(sweeps[0]), np.log(sweeps[1])))))
         if rng.random() < p_pt:
             mode, nT = 1, int(rng.integers(8, 33))

@@ -151,12 +151,81 @@ def test_batch_matches_oracle(sa, orc):
 def test_batch_rejects_bad_instances(sa):
     b = sa.Batch()
     with pytest.raises(sa.SAError):
-        b.add(2, 7, 1, 3, T=np.ones(1))  # odd L is not supported on the device
+        b.add(2, 2, 1, 3, T=np.ones(1))  # L < 3 is not supported on the device
     with pytest.raises(sa.SAError):
         b.tune(sa.TUNE_BATCH_BLOCK, 96)
     with pytest.raises(sa.SAError):
         b.tune(sa.TUNE_BATCH_FUSED, 2)  # the fused batch sweep is switched off
     b.tune(sa.TUNE_BATCH_FUSED, 1)
+    b.close()
+
+
+# odd L: the wrap groups walked inside the batch kernel
+ODD_MIX = [
+    (2, 9, 1, True, 11, 0.0),
+    (2, 5, 2, False, 14, 0.0),
+    (2, 3, 1, True, 6, 0.0),
+    (1, 15, 1, True, 17, 0.0),
+    (1, 33, 2, False, 9, 0.0),
+    (0, 13, 1, True, 12, 0.0),
+    (0, 11, 1, False, 10, 0.25),
+    (2, 7, 1, False, 8, -0.3),
+    (2, 8, 1, True, 10, 0.0),   # even instances in the same launches
+    (1, 16, 1, True, 13, 0.0),
+]
+
+
+def test_odd_L_instances_match_oracle(sa, orc):
+    """Odd-L instances of every kind, beside even ones, against the oracle's
+    counter-word sweeps."""
+    b = sa.Batch()
+    cases = [(2, 9, 7, 61), (2, 5, 4, 62), (1, 15, 9, 63), (1, 3, 5, 64), (0, 13, 6, 65), (2, 6, 5, 66)]
+    ids = [b.add(kind, L, seed, nsw, schedule=sa.linear_schedule(3.0, 0.1, nsw)) for kind, L, nsw, seed in cases]
+    assert b.run() == 0
+    for (kind, L, nsw, seed), i in zip(cases, ids):
+        say("odd-L batch vs oracle", "kind", kind, "L", L, "sweeps", nsw)
+        jx, jy, jz, s, _ = orc.init_system(kind, L, seed)
+        for t, T in enumerate(sa.linear_schedule(3.0, 0.1, nsw)):
+            orc.sweep(kind, L, s, jx, jy, jz, orc.boltz(kind, T), orc.counter_words(kind, L, seed, t))
+        assert (b.spins(i) == s).all()
+    b.close()
+
+
+def test_odd_L_mix_matches_plain_path(sa):
+    """Odd and even instances (fixed and per-sweep temperatures, fields) run in chunks
+    give the words and energies of each instance swept alone."""
+    say("odd-L mix")
+    b = sa.Batch()
+    insts = build_mix(sa, b, ODD_MIX, 77)
+    b.tune(sa.TUNE_BATCH_CHUNK, 3)
+    assert b.run() == 0
+    check_mix(sa, b, insts)
+    b.close()
+
+
+def test_odd_L_pt_matches_pt_driver(sa):
+    """Parallel tempering on odd lattices inside the batch: swap sequence and spins of
+    the PT driver run alone."""
+    b = sa.Batch()
+    pts = []
+    for q, (kind, L, nT, every, nsteps, h) in enumerate([(2, 7, 4, 5, 4, 0.0), (0, 11, 3, 6, 3, 0.2),
+                                                         (1, 9, 5, 4, 5, 0.0)]):
+        T = 0.5 * 4.0 ** (np.arange(nT)[::-1] / (nT - 1))
+        seed = 700 + q
+        pts.append((b.add_pt(kind, L, T, seed, every, nsteps, h=h), kind, L, nT, every, nsteps, h, T, seed))
+    assert b.run() == 0
+    for (i, kind, L, nT, every, nsteps, h, T, seed) in pts:
+        say("odd-L batch PT vs driver", "kind", kind, "L", L, "nT", nT)
+        ref = sa.PT(kind, L, T, h, seed, every, sa.RNG_COUNTER)
+        ref.step(nsteps)
+        r = ref.result()
+        got = b.pt_result(i)
+        assert (got["pairs"] == r["pairs"]).all()
+        assert (got["us"] == r["us"]).all()
+        assert (got["masks"] == r["masks"]).all()
+        for j in range(nT):
+            assert (b.spins(i, j) == r["spins"][j]).all(), j
+        ref.close()
     b.close()
 
 

@@ -238,7 +238,7 @@ def test_unsupported_geometry_is_an_error(sa):
         sa.Ladder(1, 2, 1)
     b = sa.Batch()
     with pytest.raises(sa.SAError):
-        b.add(2, 9, 1, 3, T=np.ones(1))  # batches run even L only
+        b.add(2, 2, 1, 3, T=np.ones(1))  # L < 3 in a batch
     b.close()
 
 

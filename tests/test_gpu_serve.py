@@ -33,7 +33,7 @@ def mixed_requests(sa, n=24, seed=3, rate=400.0):
     for q in range(n):
         t += rng.exponential(1.0 / rate)
         kind = int(rng.choice([0, 1, 2]))
-        L = int(rng.choice([8, 12, 16] if kind == 2 else [8, 16, 24, 32]))
+        L = int(rng.choice([7, 8, 12, 13, 16] if kind == 2 else [8, 15, 16, 24, 31, 32]))  # odd L too
         h = float(rng.choice([0.0, 0.0, 0.25]))
         if rng.random() < 0.5:
             r = sa.make_request(kind, L, sa.REQ_ANNEAL, int(rng.integers(20, 200)), 7000 + q, h=h, arrival=t)
@@ -110,7 +110,7 @@ def test_serve_results_equal_alone(sa):
 def test_serve_admission_order_and_refusals(sa):
     small = [sa.make_request(1, 16, sa.REQ_ANNEAL, 30, 100 + q, arrival=0.001 * q) for q in range(6)]
     big = sa.make_request(2, 32, sa.REQ_PT, 20, 99, nT=8, every=10, arrival=0.0025)
-    odd = sa.make_request(1, 15, sa.REQ_ANNEAL, 10, 98)
+    tiny = sa.make_request(1, 2, sa.REQ_ANNEAL, 10, 98)
     fp = sa.request_bytes(small[0])
     assert sa.request_bytes(big) > 2 * fp + 1024
     budget = 2 * fp + 1024  # two small requests at a time; the PT request never fits
@@ -119,7 +119,7 @@ def test_serve_admission_order_and_refusals(sa):
     ib = srv.submit(big)
     ids += [srv.submit(r) for r in small[3:]]
     with pytest.raises(sa.SAError):
-        srv.submit(odd)  # refused at submission: odd L
+        srv.submit(tiny)  # refused at submission: L < 3
     srv.run()
     res = {x.id: x for x in srv.results()}
     st = srv.stats()
