@@ -180,18 +180,27 @@ __device__ __forceinline__ uint64_t bld(buf_t b, uint32_t voff, uint32_t soff) {
   const sa_v2u32 v = __builtin_amdgcn_raw_buffer_load_b64(b, (int)voff, (int)soff, 0);
   return ((uint64_t)v.y << 32) | v.x;
 }
+// Cache policy of a load / store (gfx950: bit 1 = "nt", non-temporal).  The fused
+// sweep reads each old spin pair once and writes each new pair once per sweep, and
+// a ladder's spins are far larger than the last-level cache, so that traffic is
+// marked non-temporal; couplings (read by every system block) and the halo rows
+// (read by two bands) keep the default policy.  Measured on the 3D L=128, 32-system
+// sweep: 309 -> 283 us per launch.
+constexpr int kCpolNT = 2;
+template <int CPOL = 0>
 __device__ __forceinline__ void bld2(buf_t b, uint32_t voff, uint32_t soff, uint64_t &a, uint64_t &c) {
-  const sa_v4u32 v = __builtin_amdgcn_raw_buffer_load_b128(b, (int)voff, (int)soff, 0);
+  const sa_v4u32 v = __builtin_amdgcn_raw_buffer_load_b128(b, (int)voff, (int)soff, CPOL);
   a = ((uint64_t)v.y << 32) | v.x;
   c = ((uint64_t)v.w << 32) | v.z;
 }
+template <int CPOL = 0>
 __device__ __forceinline__ void bst2(buf_t b, uint32_t voff, uint32_t soff, uint64_t a, uint64_t c) {
   sa_v4u32 v;
   v.x = (uint32_t)a;
   v.y = (uint32_t)(a >> 32);
   v.z = (uint32_t)c;
   v.w = (uint32_t)(c >> 32);
-  __builtin_amdgcn_raw_buffer_store_b128(v, b, (int)voff, (int)soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b128(v, b, (int)voff, (int)soff, CPOL);
 }
 
 // ---------------------------------------------------------------- 3D half-sweep
@@ -382,7 +391,7 @@ struct fused_ctx {
     const uint32_t vn = top ? kOutOfRange : vpair;
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
-      bld2(sin[m], vn, Pd, q.na[m], q.nb[m]);
+      bld2<kCpolNT>(sin[m], vn, Pd, q.na[m], q.nb[m]);
       q.ring2[m] = bld(sin[m], vring, P);
     }
     bld2(bjx, vpair, P, q.jxa, q.jxb);
@@ -590,7 +599,7 @@ struct fused_ctx {
           for (int m = 0; m < MB; ++m) oa[m] = ob[m] = 0;
         }
 #pragma unroll
-        for (int m = 0; m < MB; ++m) bst2(sout[m], vstore, (uint32_t)z * plane_bytes, oa[m], ob[m]);
+        for (int m = 0; m < MB; ++m) bst2<kCpolNT>(sout[m], vstore, (uint32_t)z * plane_bytes, oa[m], ob[m]);
       }
       if (z + 1 < L) {
         uint64_t g0, g1;
