@@ -179,7 +179,8 @@ def test_odd_L_instances_match_oracle(sa, orc):
     """Odd-L instances of every kind, beside even ones, against the oracle's
     counter-word sweeps."""
     b = sa.Batch()
-    cases = [(2, 9, 7, 61), (2, 5, 4, 62), (1, 15, 9, 63), (1, 3, 5, 64), (0, 13, 6, 65), (2, 6, 5, 66)]
+    cases = [(2, 9, 7, 61), (2, 5, 4, 62), (1, 15, 9, 63), (1, 3, 5, 64), (0, 13, 6, 65), (2, 6, 5, 66),
+             (2, 37, 3, 67)]
     ids = [b.add(kind, L, seed, nsw, schedule=sa.linear_schedule(3.0, 0.1, nsw)) for kind, L, nsw, seed in cases]
     assert b.run() == 0
     for (kind, L, nsw, seed), i in zip(cases, ids):

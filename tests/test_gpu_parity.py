@@ -391,10 +391,13 @@ def test_replay_fused_L128_matches_oracle(sa, orc, mb):
 
 # ---- odd L: the wrap neighbours share a colour --------------------------------------------
 @pytest.mark.parametrize("kind,L,nsys", [(2, 3, 1), (2, 5, 2), (2, 7, 1), (2, 9, 3), (2, 15, 1), (1, 3, 1),
-                                         (1, 5, 1), (1, 9, 2), (1, 21, 1), (0, 5, 1), (0, 7, 2), (0, 13, 1)])
+                                         (1, 5, 1), (1, 9, 2), (1, 21, 1), (0, 5, 1), (0, 7, 2), (0, 13, 1),
+                                         # 3D: the lane pairs span two waves (L >= 35), the largest
+                                         # served 3D size; 2D: the largest served size
+                                         (2, 35, 1), (2, 63, 2), (1, 129, 1), (0, 255, 1)])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_odd_L_matches_oracle(sa, orc, kind, L, nsys, mode):
-    """Odd L (one thread per wrap group, sites in the reference's visit order) against
+    """Odd L (3D: one lane per wrap-group member; 2D: one thread per group) against
     the oracle, which the compiled reference pins for odd L as well: three sweeps, per
     system its own temperature, a field on the even systems of the counter runs."""
     say("odd L", "kind", kind, "L", L, "nsys", nsys, "mode", mode)
