@@ -4,9 +4,13 @@
 // Reference: src/gpu_sys.cuh:12-29 (gpu_sys) and src/cpu_sys.hpp:12-31 (cpu_sys,
 // which adds get_T / clone / a virtual destructor; they are added here too, the
 // reference's missing virtual destructor being the one change of substance).
+// get_T and clone have default bodies, so a class written against the reference's
+// gpu_sys (which has neither) still compiles: get_T() then reports NaN and clone()
+// nullptr, which parallel_tempering::update_T reports as an error.
 #pragma once
 
 #include <iostream>
+#include <limits>
 #include <memory>
 #include <string>
 #include <vector>
@@ -30,7 +34,7 @@ class gpu_sys {
   virtual void swap(gpu_sys *sys, std::unique_ptr<spin_t[]> mask) = 0;
   virtual std::ostream &save(std::ostream &stream) = 0;
   virtual std::istream &load(std::istream &stream) = 0;
-  virtual double get_T() = 0;
-  virtual gpu_sys *clone() const = 0;
+  virtual double get_T() { return std::numeric_limits<double>::quiet_NaN(); }
+  virtual gpu_sys *clone() const { return nullptr; }
   virtual ~gpu_sys() = default;
 };

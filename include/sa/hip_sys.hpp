@@ -64,6 +64,11 @@ class ladder {
     M.resize((size_t)nsys() * 64);
     return sa_ladder_measure(l_, E.data(), M.data());
   }
+  // Set when a system record loaded into one slot wrote the state all slots share
+  // (couplings, engine, counter stream): a later slot's record must agree with it.
+  // parallel_tempering::load clears it at the start of a checkpoint.
+  bool shared_loaded = false;
+  uint64_t shared_rng = 0;  // the random source that record set (sa_rng_mode)
 
  private:
   sa_ladder *l_ = nullptr;
@@ -265,6 +270,16 @@ class hip_sys : public gpu_sys {
       is.clear();
       is.seekg(mark);
     }
+    // Slots of one ladder share couplings and the random stream.  A record that
+    // disagrees with what an earlier slot's record put there (a system inserted by
+    // update_T restarts its engine at seed 1234, src/cpu_3d.cpp:47-80, and the
+    // reference restores every system's engine, src/parallel_tempering.hpp:362-372)
+    // moves this system onto a ladder of its own instead of overwriting the others'.
+    if (lad_->nsys() > 1 && lad_->shared_loaded && !matches_shared(j, eng, trailer)) {
+      auto own = std::make_shared<ladder>(lad_->kind(), lad_->L(), 1);
+      lad_ = std::move(own);
+      idx_ = 0;
+    }
     T_ = T;
     h_ = h;
     apply_T();
@@ -277,6 +292,8 @@ class hip_sys : public gpu_sys {
       check("load", sa_ladder_set_sweep_index(l(), trailer[3]));
     }
     check("load", sa_ladder_set_engine(l(), eng));
+    lad_->shared_loaded = true;
+    lad_->shared_rng = trailer[1];
     return is;
   }
   double get_T() override { return T_; }
@@ -324,6 +341,27 @@ class hip_sys : public gpu_sys {
     return false;
   }
   void apply_T() { check("set_T", sa_ladder_set_temperature(l(), idx_, T_, h_, nullptr)); }
+  // does a loaded record's shared state (deduplicated couplings j, engine, counter
+  // trailer) equal the ladder's?
+  bool matches_shared(const std::vector<uint64_t> (&j)[3], const uint64_t (&eng)[313],
+                      const uint64_t (&trailer)[4]) const {
+    uint64_t cur[313];
+    if (sa_ladder_get_engine(l(), cur) != SA_OK || std::memcmp(cur, eng, sizeof cur) != 0) return false;
+    if (lad_->shared_rng != trailer[1]) return false;
+    if (trailer[1] != SA_RNG_REPLAY) {
+      uint64_t seed = 0, t = 0;
+      sa_ladder_get_seed(l(), &seed);
+      sa_ladder_get_sweep_index(l(), &t);
+      if (seed != trailer[2] || t != trailer[3]) return false;
+    }
+    const long N = lad_->N();
+    std::vector<uint64_t> c[3];
+    for (auto &a : c) a.resize(N);
+    if (sa_ladder_get_couplings(l(), c[0].data(), c[1].data(), c[2].data()) != SA_OK) return false;
+    for (int a = 0; a < dim(); ++a)
+      if (c[a] != j[a]) return false;
+    return true;
+  }
   // couplings in the reference's duplicated layout: [2i] = own bond, [2i+1] = the
   // +axis neighbour's bond (src/cpu_3d.cpp:290-299)
   std::vector<std::vector<uint64_t>> duplicated_couplings() const {

@@ -183,10 +183,14 @@ class parallel_tempering {
     for (size_t i = 0; i < num_switch_.size(); i++) {
       if (num_switch_[i].get() < 0.3) {
         i++;
+        system *c = static_cast<system *>(sys_[i]->clone());
+        if (!c) {  // a system class without clone() (gpu_sys's default body)
+          std::cerr << "Error:   update_T: system " << i << " cannot be cloned; no temperature inserted\n";
+          return;
+        }
         num_switch_.insert(num_switch_.begin() + i, accumulator{});
         T_.insert(T_.begin() + i, (T_[i] + T_[i - 1]) / 2.0);
         E_.insert(E_.begin() + i, std::vector<float>());
-        system *c = static_cast<system *>(sys_[i]->clone());
         sys_.insert(sys_.begin() + i, c);
         owned_.emplace_back(c);
         sys_[i]->set_T(T_[i]);
@@ -262,6 +266,15 @@ class parallel_tempering {
     num_switch_.assign(n, accumulator{});
     r(num_switch_.data(), n * sizeof(accumulator));
     r(&n, sizeof n);
+    if (n != sys_.size()) {  // the reference reads one record per system it has
+      std::cerr << "Error:   load: the checkpoint holds " << n << " systems, the driver " << sys_.size() << "\n";
+      is.setstate(std::ios::failbit);
+      return is;
+    }
+    // a new checkpoint: every shared ladder takes its state from the first of its
+    // slots' records again (hip_sys::load)
+    for (system *s : sys_)
+      if (hip_sys *h = as_hip(s)) h->get_ladder()->shared_loaded = false;
     E_.assign(n, std::vector<float>());
     for (size_t i = 0; i < n && i < sys_.size(); ++i) {
       size_t m = 0;

@@ -28,10 +28,10 @@ _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 
 
 def build(force: bool = False) -> str:
-    """Compile oracle.c into liboracle.so (plain gcc -O2, like the reference's makefile)."""
+    """Compile oracle.c into liboracle.so (gcc -O2 -fopenmp, like the reference's makefile)."""
     src = os.path.join(HERE, "oracle.c")
     if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
-        subprocess.check_call(["gcc", "-O2", "-std=c99", "-fPIC", "-shared", "-o", LIB_PATH, src, "-lm"])
+        subprocess.check_call(["gcc", "-O2", "-std=c99", "-fopenmp", "-fPIC", "-shared", "-o", LIB_PATH, src, "-lm"])
     return LIB_PATH
 
 

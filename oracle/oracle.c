@@ -380,7 +380,7 @@ int or_pt_run(int kind, int L, int nT, const double *Tin, double h, uint64_t see
               uint64_t *masks, float *E_last, uint64_t *spins) {
   if (nT < 2) return -1;
   int64_t N = (int64_t)L * L * ((kind == OR_3D) ? L : 1);
-  uint64_t *jx = malloc(N * 8), *jy = malloc(N * 8), *jz = malloc(N * 8), *words = malloc(N * 8);
+  uint64_t *jx = malloc(N * 8), *jy = malloc(N * 8), *jz = malloc(N * 8);
   or_member *m = calloc((size_t)nT, sizeof(or_member));
   float *E = calloc((size_t)nT * 64, sizeof(float));
   double *T = malloc(nT * sizeof(double));
@@ -403,11 +403,20 @@ int or_pt_run(int kind, int L, int nT, const double *Tin, double h, uint64_t see
   or_mt32_seed(&pg, 123);
   for (int step = 0; step < nsteps; ++step) {
     /* step(0): sweeps, measure on the last one (src/parallel_tempering.hpp:81-115) */
-    for (int i = 0; i < sweeps; ++i)
-      for (int j = 0; j < nT; ++j) {
-        member_sweep(kind, L, &m[j], jx, jy, jz, rng_mode, seed, words);
-        if (i == sweeps - 1) or_measure_ref_order(kind, L, m[j].s, jx, jy, jz, h, &E[j * 64]);
+    for (int i = 0; i < sweeps; ++i) {
+      /* the members are independent until the exchange: one OpenMP thread per
+       * member, as the reference's step does (src/parallel_tempering.hpp:86) */
+#pragma omp parallel
+      {
+        uint64_t *w = malloc(N * 8);
+#pragma omp for schedule(dynamic)
+        for (int j = 0; j < nT; ++j) {
+          member_sweep(kind, L, &m[j], jx, jy, jz, rng_mode, seed, w);
+          if (i == sweeps - 1) or_measure_ref_order(kind, L, m[j].s, jx, jy, jz, h, &E[j * 64]);
+        }
+        free(w);
       }
+    }
     int i = or_mt32_uniform_int(&pg, 0, nT - 2);
     double u = or_mt32_uniform(&pg);
     uint64_t mask = or_pt_mask(&E[i * 64], &E[(i + 1) * 64], T[i], T[i + 1], u);
@@ -420,7 +429,7 @@ int or_pt_run(int kind, int L, int nT, const double *Tin, double h, uint64_t see
   if (spins)
     for (int j = 0; j < nT; ++j) memcpy(spins + (int64_t)j * N, m[j].s, N * 8);
   for (int j = 0; j < nT; ++j) free(m[j].s);
-  free(m); free(E); free(T); free(jx); free(jy); free(jz); free(words);
+  free(m); free(E); free(T); free(jx); free(jy); free(jz);
   return 0;
 }
 

@@ -828,11 +828,25 @@ struct sa_detail::prepared_instance {
 };
 
 namespace sa_detail {
+int batch_geometry(int kind, int L, int nsys) {
+  if (kind < SA_KIND_2P || kind > SA_KIND_3D) return sa_fail(SA_ERR_ARG, "unknown kind %d", kind);
+  if (nsys < 1) return sa_fail(SA_ERR_ARG, "nsys must be >= 1");
+  if (L < 3) return sa_fail(SA_ERR_UNSUPPORTED, "device path needs L >= 3 (got %d)", L);
+  if ((kind == SA_KIND_3D && L > 1024) || (kind != SA_KIND_3D && L > 32768))
+    return sa_fail(SA_ERR_ARG, "L=%d too large", L);
+  // k_batch_half and the block map carry an instance's work item in an int
+  if ((int64_t)half_items(kind, L) * nsys >= ((int64_t)1 << 31))
+    return sa_fail(SA_ERR_UNSUPPORTED, "instance too large for the batch: %lld work items per half-sweep (limit 2^31)",
+                   (long long)half_items(kind, L) * nsys);
+  return SA_OK;
+}
+
 int batch_prepare(int kind, int L, int nsys, uint64_t seed, int nsweeps, const double *T, int per_sweep, double h,
                   int pt_every, prepared_instance **out) {
   *out = nullptr;
+  int rc = batch_geometry(kind, L, nsys);
+  if (rc) return rc;
   std::unique_ptr<prepared_instance> p(new prepared_instance());
-  int rc;
   if (pt_every > 0) {
     if (nsys < 2 || nsweeps % pt_every) return sa_fail(SA_ERR_ARG, "bad parallel-tempering instance");
     pt_state pt = make_pt(nsys, T, h, pt_every);
@@ -978,12 +992,16 @@ int sa_batch_destroy(sa_batch *b) {
 
 int sa_batch_add(sa_batch *b, int kind, int L, int nsys, uint64_t seed, int nsweeps, const double *T, int per_sweep,
                  double h, int *id) {
-  if (!b || !T || !id || nsweeps < 0 || nsys < 1) return sa_fail(SA_ERR_ARG, "bad argument");
+  const int rc = sa_detail::batch_geometry(kind, L, nsys);  // before anything touches the device
+  if (rc) return rc;
+  if (!b || !T || !id || nsweeps < 0) return sa_fail(SA_ERR_ARG, "bad argument");
   return add_instance(b, kind, L, nsys, seed, nsweeps, T, per_sweep, h, nullptr, id);
 }
 
 int sa_batch_add_pt(sa_batch *b, int kind, int L, int nT, const double *T, double h, uint64_t seed, int every,
                     int nsteps, int *id) {
+  const int rc = sa_detail::batch_geometry(kind, L, nT);  // before anything touches the device
+  if (rc) return rc;
   if (!b || !T || !id || nT < 2 || every < 1 || nsteps < 0) return sa_fail(SA_ERR_ARG, "bad argument");
   if ((int64_t)every * nsteps > INT32_MAX / 2) return sa_fail(SA_ERR_ARG, "too many sweeps");
   pt_state p = make_pt(nT, T, h, every);

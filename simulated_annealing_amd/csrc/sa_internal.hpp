@@ -55,6 +55,10 @@ void make_table(int kind, double T, double h, sa::sys_table &tab, bool &fast, do
 // device bytes sa_ladder_create allocates for a ladder (the fused 3D sweep's second
 // spin array, allocated on first use, not included)
 int64_t ladder_bytes(int kind, int L, int nsys);
+// SA_OK when an instance of nsys systems of (kind, L) fits the batch kernels: the
+// batch indexes an instance's work items (half_items(kind, L) * nsys) in 32 bits;
+// otherwise an sa_status (recorded for sa_last_error), checked before any device work
+int batch_geometry(int kind, int L, int nsys);
 // device bytes of a batch instance: its ladder plus its temperature tables
 int64_t batch_instance_bytes(int kind, int L, int nsys, int nsweeps, int per_sweep);
 // One chunk of a batch in two halves, so a caller can do host work while it runs:

@@ -67,7 +67,7 @@ int check_request(const sa_request &r) {
     return sa_fail(SA_ERR_ARG, "request: unknown mode %d", r.mode);
   }
   if (!(r.T0 > 0) || !(r.T1 > 0)) return sa_fail(SA_ERR_ARG, "request: temperatures must be positive");
-  return SA_OK;
+  return sa_detail::batch_geometry(r.kind, r.L, r.mode == SA_REQ_PT ? r.nT : 1);
 }
 
 }  // namespace

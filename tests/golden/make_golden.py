@@ -6,6 +6,7 @@ sources).  Run in the container that has the reference sources:
     python tests/golden/make_golden.py --checkpoints   # ckpt_vectors.json + ckpt_*.tempsys
     python tests/golden/make_golden.py --pt-files      # pt_files.json
     python tests/golden/make_golden.py --ground        # pt_ground.json
+    python tests/golden/make_golden.py --configs       # config_vectors.json
 
 The fixtures are data only: inputs, seeds and the reference's outputs.  Spin arrays
 are stored whole (hex words) for small lattices and as SHA-256 of their
@@ -196,9 +197,61 @@ def ground():
     return out
 
 
+CONFIG_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "config_vectors.json")
+
+
+def config_runs():
+    """The benchmark configurations at their stated sizes, from the reference's own
+    classes and drivers (replay stream, seed 1234):
+      * 2D periodic L=64, linear anneal T_k = 3.0 - 2.9 k/999 over 1,000 sweeps
+        (set_T before every sweep), 64 energies after every sweep k with k % 10 == 0;
+      * 3D L=64 at T=1.0, 10,000 sweeps: energies every 1,000 sweeps, final spins;
+      * 3D L=128 parallel tempering over 32 temperatures geometric 0.5 .. 2.0,
+        10 sweeps per exchange attempt, 3 steps: pair, uniform and mask per step,
+        the energies of all 32 temperatures and every system's spins."""
+    out = {}
+    kind, L, seed, n, stride = 1, 64, 1234, 1000, 10
+    Ts = [3.0 - 2.9 * k / (n - 1) for k in range(n)]
+    r = O.RefSystem(kind, L, Ts[0])
+    r.seed_like_main(seed)
+    recs = []
+    for k in range(n):
+        r.set_T(Ts[k])
+        r.sweep(1)
+        if k % stride == 0:
+            recs.append(f32bits(r.measure()))
+    out["anneal2d_L64"] = dict(kind=kind, L=L, seed=seed, T=Ts, stride=stride, E=recs,
+                               E_final=f32bits(r.measure()), s_sha=sha(r.state()["s"]))
+    print("config 1 done", flush=True)
+    kind, L, T, n, every = 2, 64, 1.0, 10000, 1000
+    r = O.RefSystem(kind, L, T)
+    r.seed_like_main(seed)
+    recs, shas = [], []
+    for k in range(n // every):
+        r.sweep(every)
+        recs.append(f32bits(r.measure()))
+        shas.append(sha(r.state()["s"]))
+        print(f"config 2: {(k + 1) * every} sweeps", flush=True)
+    out["fixed3d_L64"] = dict(kind=kind, L=L, seed=seed, T=T, sweeps=n, every=every, E=recs, s_sha=shas)
+    kind, L, nT, sweeps, nsteps = 2, 128, 32, 10, 3
+    T = [0.5 * 4.0 ** (k / (nT - 1)) for k in range(nT)]
+    p = O.ref_pt_run(kind, L, np.array(T), 0.0, seed, sweeps, nsteps)
+    out["pt3d_L128"] = dict(kind=kind, L=L, T=T, h=0.0, seed=seed, sweeps=sweeps, nsteps=nsteps,
+                            pairs=[int(x) for x in p["pairs"]], us=f64bits(p["us"]), masks=hexw(p["masks"]),
+                            E=[f32bits(e) for e in p["E"]], spins_sha=[sha(s) for s in p["spins"]])
+    print("config 3 done", flush=True)
+    return out
+
+
 def main():
     if not O.ref_available():
         O.build_ref()
+    if "--configs" in sys.argv:  # only the benchmark-configuration fixtures
+        with open(CONFIG_OUT, "w") as f:
+            json.dump(dict(about="The benchmark configurations run by the reference (see make_golden.py).",
+                           **config_runs()), f, indent=0, separators=(",", ":"))
+        print("wrote", CONFIG_OUT)
+        return
     if "--ground" in sys.argv:  # only the ground-state loop fixtures
         with open(GROUND_OUT, "w") as f:
             json.dump(dict(about="update_T / reset / get_E_min of the reference PT driver (see make_golden.py).",

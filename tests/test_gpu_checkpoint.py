@@ -84,3 +84,32 @@ def test_counter_rng_round_trip(sa, tmp_path):
     assert (ra["spins"] == rb["spins"]).all()
     a.close()
     b.close()
+
+
+def test_load_after_update_T_into_one_ladder(sa, tmp_path):
+    """A checkpoint written after update_T holds inserted systems whose engines
+    restarted at seed 1234 (the reference's clone): loading it into a driver whose
+    systems are slots of one ladder must not overwrite the shared engine with theirs
+    -- those systems move onto ladders of their own -- and the run continues as the
+    original does (replay stream, compared step by step)."""
+    say("load after update_T")
+    T = [0.4, 1.2, 3.0]
+    a = sa.PT(2, 8, T, 0.0, 77, 5, sa.RNG_REPLAY)
+    a.step(20)
+    n = a.update_temperatures()
+    assert n > len(T)
+    a.step(3)
+    a.save(tmp_path / "u.tempsys")
+    a.step(6)
+    ra = a.result()
+    b = sa.PT(2, 8, a.temperatures(), 0.0, 1, 5, sa.RNG_REPLAY)
+    b.load(tmp_path / "u.tempsys")
+    b.step(6)
+    rb = b.result()
+    assert (ra["pairs"][-6:] == rb["pairs"]).all()
+    assert (ra["us"][-6:] == rb["us"]).all()
+    assert (ra["masks"][-6:] == rb["masks"]).all()
+    assert (ra["spins"] == rb["spins"]).all()
+    assert (ra["E"] == rb["E"]).all()
+    a.close()
+    b.close()
