@@ -127,6 +127,20 @@ int sa_ladder_measure(sa_ladder *l, int32_t *E, int32_t *M);
  * (src/parallel_tempering.hpp:91-114). */
 int sa_ladder_measure_systems(sa_ladder *l, const int *sys, int n, int32_t *E, int32_t *M);
 
+/* sa_ladder_measure without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device or host memory (code under HIP graph
+ * capture, serving loops and harnesses that forbid hipMalloc).  The caller passes
+ * device arrays: s (nsys*N words, system-major), jx/jy/jz (N words each; jz unused
+ * in 2D) and receives E[sys*64+k], M[sys*64+k] (nsys*64 int32 each, device memory)
+ * with the values sa_ladder_measure returns.  Asynchronous on `stream` (a
+ * hipStream_t; NULL = the null stream).  The partial counters live in one fixed
+ * device buffer per GPU, sized for at most SA_NOALLOC_MAX_SYSTEMS systems (256 KiB,
+ * held until the process ends); larger nsys returns SA_ERR_ARG.  Every call on a GPU
+ * uses that GPU's buffer, so calls on one GPU that overlap in time race. */
+#define SA_NOALLOC_MAX_SYSTEMS 64
+int sa_measure_noalloc(int kind, int L, int nsys, const uint64_t *s, const uint64_t *jx, const uint64_t *jy,
+                       const uint64_t *jz, int32_t *E, int32_t *M, void *stream);
+
 /* Exchange the lanes selected by `mask` between systems a and b
  * (swap, src/gpu_3d.cu:310-328, src/cpu_3d.cpp:429-444). */
 int sa_ladder_swap(sa_ladder *l, int a, int b, uint64_t mask);

@@ -31,7 +31,7 @@ EXPORTS = [
     "sa_ladder_init_couplings", "sa_ladder_init_spins", "sa_ladder_set_couplings", "sa_ladder_get_couplings",
     "sa_ladder_set_spins", "sa_ladder_get_spins", "sa_ladder_set_temperature", "sa_ladder_sweep",
     "sa_ladder_anneal",
-    "sa_ladder_measure", "sa_ladder_measure_systems", "sa_ladder_swap", "sa_ladder_sync", "sa_ladder_stream",
+    "sa_ladder_measure", "sa_ladder_measure_systems", "sa_measure_noalloc", "sa_ladder_swap", "sa_ladder_sync", "sa_ladder_stream",
     "sa_ladder_set_tuning", "sa_ladder_get_tuning",
     "sa_pt_run", "sa_anneal_run", "sa_anneal_geometric",
     "sa_pt_create", "sa_pt_step", "sa_pt_sync", "sa_pt_steps_done", "sa_pt_result", "sa_pt_ladder",
@@ -119,6 +119,7 @@ def lib():
         L.sa_ladder_anneal.argtypes = [_vp, _f64p, C.c_int, C.c_double]
         L.sa_ladder_measure.argtypes = [_vp, _i32p, _i32p]
         L.sa_ladder_measure_systems.argtypes = [_vp, _i32p, C.c_int, _i32p, _i32p]
+        L.sa_measure_noalloc.argtypes = [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
         L.sa_ladder_swap.argtypes = [_vp, C.c_int, C.c_int, C.c_uint64]
         L.sa_ladder_sync.argtypes = [_vp]
         L.sa_ladder_stream.argtypes = [_vp]

@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["sa_ladder.hip", "sa_batch.hip", "sa_serve.cpp", "sa_drivers.cpp"]
+SOURCES = ["sa_ladder.hip", "sa_batch.hip", "sa_noalloc.hip", "sa_serve.cpp", "sa_drivers.cpp"]
 
 
 def hipcc() -> str:

@@ -52,6 +52,13 @@ namespace sa_detail {
 // acceptance thresholds of one temperature/field for a ladder of `kind`; fast =
 // the table qualifies for the symmetric low-levels-always kernels
 void make_table(int kind, double T, double h, sa::sys_table &tab, bool &fast, double *boltz_out);
+// Measurement of n systems of s (nsys x N words; listed by dlist in device memory,
+// by sel for up to four, else systems 0..n-1): clears cnt (kMeasureCopies x n x 128
+// ints) with a fill kernel, then per copy, listed system and lane the satisfied
+// -x,-y[,-z] bonds (cnt[..][j][0][lane]) and set spins (cnt[..][j][1][lane]).
+// Asynchronous on st; the caller sums the copies.
+void launch_measure(int kind, int L, int64_t N, const uint64_t *s, const uint64_t *jx, const uint64_t *jy,
+                    const uint64_t *jz, int n, const int *dlist, sa::sys4 sel, int *cnt, hipStream_t st);
 // device bytes sa_ladder_create allocates for a ladder (the fused 3D sweep's second
 // spin array, allocated on first use, not included)
 int64_t ladder_bytes(int kind, int L, int nsys);

@@ -457,6 +457,37 @@ void sa_detail::make_table(int kind, double T, double h, sa::sys_table &t, bool 
   fast = fast && (t.always[0] & low) == low && (t.always[0] >> (nb / 2 + 1)) == 0;
 }
 
+void sa_detail::launch_measure(int kind, int L, int64_t N, const uint64_t *s, const uint64_t *jx, const uint64_t *jy,
+                               const uint64_t *jz, int n, const int *dlist, sa::sys4 sel, int *cnt, hipStream_t st) {
+  const int all_n = 128 * n * sa::kMeasureCopies;
+  hipLaunchKernelGGL(sa::k_fill_zero, dim3((all_n + 255) / 256), dim3(256), 0, st, cnt, all_n);
+  // cnt layout [copy][listed system][2][64] (the caller sums the copies); one block
+  // row per SB systems (4, or 1 when there are at most 4: more waves in flight for
+  // the few-system measurement of an exchange step).  A wave converts its bit-sliced
+  // counters (a 64x64 transpose per plane) after 16 site groups or at its end, so
+  // each wave gets at least 16 groups where possible (32 with one system per row,
+  // four of them loaded at a time): more blocks would only add conversions.
+  const int sb = n <= 4 ? 1 : 4;
+  const int yb = (n + sb - 1) / sb;
+  const int64_t want = std::max<int64_t>(1, (sb == 1 ? 4096 : 2048) / yb);
+  const int64_t per = sb == 1 ? 32 : 16;  // site groups per wave (one system per row: 4 in flight at a time)
+  const int64_t blocks = std::min<int64_t>((N + 256 * per - 1) / (256 * per), want);
+  dim3 grid((unsigned)blocks, yb);
+#define SA_MEASURE(KINDV)                                                                                        \
+  if (sb == 1)                                                                                                   \
+    hipLaunchKernelGGL((sa::k_measure<KINDV, 1>), grid, dim3(256), 0, st, s, jx, jy, jz, L, N, n, dlist, sel, cnt); \
+  else                                                                                                           \
+    hipLaunchKernelGGL((sa::k_measure<KINDV, 4>), grid, dim3(256), 0, st, s, jx, jy, jz, L, N, n, dlist, sel, cnt);
+  if (kind == SA_KIND_3D) {
+    SA_MEASURE(sa::KIND_3D)
+  } else if (kind == SA_KIND_2D) {
+    SA_MEASURE(sa::KIND_2D)
+  } else {
+    SA_MEASURE(sa::KIND_2P)
+  }
+#undef SA_MEASURE
+}
+
 extern "C" {
 
 int sa_ladder_set_temperature(sa_ladder *l, int sys, double T, double h, double *boltz_out) {
@@ -512,7 +543,6 @@ namespace {
 // land in cnt_h[(2*j)*64 ...] for the j-th listed system
 int measure_list(sa_ladder *l, const int *list, int n) {
   const int cnt_n = 128 * n, all_n = cnt_n * sa::kMeasureCopies;
-  hipLaunchKernelGGL(sa::k_fill_zero, dim3((all_n + 255) / 256), dim3(256), 0, l->stream, l->cnt, all_n);
   // up to 4 listed systems travel as a kernel argument; more through device memory
   const int *dlist = nullptr;
   sa::sys4 sel{{-1, -1, -1, -1}};
@@ -522,33 +552,7 @@ int measure_list(sa_ladder *l, const int *list, int n) {
     SA_HIP_TRY(hipMemcpyAsync(l->sys_list, list, sizeof(int) * n, hipMemcpyHostToDevice, l->stream));
     dlist = l->sys_list;
   }
-  // cnt layout [copy][listed system][2][64] (the copies are summed below); one block
-  // row per SB systems (4, or 1 when there are at most 4: more waves in flight for
-  // the few-system measurement of an exchange step).  A wave converts its bit-sliced
-  // counters (a 64x64 transpose per plane) after 16 site groups or at its end, so
-  // each wave gets at least 16 groups where possible (32 with one system per row,
-  // four of them loaded at a time): more blocks would only add conversions.
-  const int sb = n <= 4 ? 1 : 4;
-  const int yb = (n + sb - 1) / sb;
-  const int64_t want = std::max<int64_t>(1, (sb == 1 ? 4096 : 2048) / yb);
-  const int64_t per = sb == 1 ? 32 : 16;  // site groups per wave (one system per row: 4 in flight at a time)
-  const int64_t blocks = std::min<int64_t>((l->N + 256 * per - 1) / (256 * per), want);
-  dim3 grid((unsigned)blocks, yb);
-#define SA_MEASURE(KINDV)                                                                                  \
-  if (sb == 1)                                                                                             \
-    hipLaunchKernelGGL((sa::k_measure<KINDV, 1>), grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, \
-                       l->L, l->N, n, dlist, sel, l->cnt);                                                      \
-  else                                                                                                     \
-    hipLaunchKernelGGL((sa::k_measure<KINDV, 4>), grid, dim3(256), 0, l->stream, l->s, l->jx, l->jy, l->jz, \
-                       l->L, l->N, n, dlist, sel, l->cnt);
-  if (l->kind == SA_KIND_3D) {
-    SA_MEASURE(sa::KIND_3D)
-  } else if (l->kind == SA_KIND_2D) {
-    SA_MEASURE(sa::KIND_2D)
-  } else {
-    SA_MEASURE(sa::KIND_2P)
-  }
-#undef SA_MEASURE
+  sa_detail::launch_measure(l->kind, l->L, l->N, l->s, l->jx, l->jy, l->jz, n, dlist, sel, l->cnt, l->stream);
   SA_HIP_TRY(hipGetLastError());
   // The counters land in pinned memory allocated at the first measurement (so
   // creating a ladder, e.g. per served request, needs no pinned allocation) and the
