@@ -127,8 +127,12 @@ def batched_vs_plain(sa, n=512):
         lad.sync()
     dt_p = time.perf_counter() - t0
     same = sum(int((b.spins(i) == lad.spins(0)).all()) for i, lad in zip(ids, lads))
+    # compulsory bytes of the batched run: per site-sweep 16 B of spins (read + write)
+    # and the site's 2 (2D) or 3 (3D) coupling words, one system per instance
+    alg = sum(w.sites * w.nsweeps * (16 + 8 * (3 if w.kind == sa.KIND_3D else 2)) for w in W)
     out = dict(workload=f"{n} uneven instances: 60% 2D L in {{32..128}}, 40% 3D L in {{16..48}}, "
                         "linear anneal 3.0->0.1 over 500/1000/2000 sweeps",
+               batched_roofline_frac=round(alg / dt_b / (HBM_PEAK_GBS * 1e9), 4), alg_bytes=alg,
                attempts=attempts, batched_attempts_per_ns=round(attempts / dt_b / 1e9, 1),
                plain_attempts_per_ns=round(attempts / dt_p / 1e9, 1), speedup=round(dt_p / dt_b, 2),
                batched_s=round(dt_b, 4), plain_s=round(dt_p, 4), bitexact_instances=f"{same}/{n}",
@@ -137,6 +141,29 @@ def batched_vs_plain(sa, n=512):
         lad.close()
     b.close()
     return out
+
+
+def single_lattice(sa, L1=64, sweeps=2000):
+    """One 3D L=64 lattice at T=1.0 (64 samples per word, counter RNG, seeded like
+    main): sweeps of a single system, which offers L*L/2 columns per half-sweep.
+    Roofline basis: 40 B per word-sweep (16 B spins read + written, 24 B couplings)."""
+    lad = sa.Ladder(sa.KIND_3D, L1, 1)
+    lad.set_rng(sa.RNG_COUNTER)
+    lad.seed_like_main(1234)
+    lad.set_temperature(0, 1.0)
+    lad.sweep(50)
+    lad.sync()
+    t0 = time.perf_counter()
+    lad.sweep(sweeps)
+    lad.sync()
+    dt = time.perf_counter() - t0
+    N1 = L1**3
+    tune = {k: lad.tuning(v) for k, v in (("kernel", sa.TUNE_SWEEP3D_KERNEL), ("rows_per_tile", sa.TUNE_ROWS_PER_TILE))}
+    lad.close()
+    us = dt / sweeps * 1e6
+    return dict(workload=f"3D +-J EA L={L1}, one system at T=1.0, {sweeps} sweeps, counter RNG",
+                attempts_per_ns=round(N1 * 64 * sweeps / dt / 1e9, 1), us_per_sweep=round(us, 2),
+                roofline_frac=round(N1 * 40 / (us * 1e-6) / (HBM_PEAK_GBS * 1e9), 4), tuning=tune)
 
 
 def serving(sa, n=1000, rate=40.0, budget=0, max_active=0):
@@ -170,8 +197,9 @@ def serving(sa, n=1000, rate=40.0, budget=0, max_active=0):
     wait = np.array([x.admitted - x.arrival for x in ok])
     span = max(x.finished for x in ok) - min(x.arrival for x in ok)
     attempts = sum(x.attempts for x in ok)
+    reduced = "" if n >= 20000 else f" (the first {n} of the 20,000-request stream, so the default run ends in minutes)"
     return dict(workload=f"{n} Poisson requests at {rate:g}/s: 2D L~logU[16,256] / 3D L~logU[8,64], "
-                         "anneal or PT (8..32 T), sweeps~logU[100,10000]",
+                         f"anneal or PT (8..32 T), sweeps~logU[100,10000]{reduced}",
                 requests=n, completed=len(ok), rate_per_s=rate,
                 p50_latency_s=round(float(np.percentile(lat, 50)), 4),
                 p99_latency_s=round(float(np.percentile(lat, 99)), 4),
@@ -190,6 +218,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-single", action="store_true", help="skip the single-lattice L=64 measurement")
     ap.add_argument("--serve-requests", type=int, default=1000, help="requests in the serving measurement")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-only", action="store_true", help="print only the serving measurement")
@@ -257,14 +286,11 @@ def main():
             prof = json.load(f)
         if prof.get("kernel") == roofline_kernel(fused):
             traffic = prof.get("hbm_bytes_per_launch")
-    # the same launches on the single-lattice basis (40 B per word-sweep: couplings
-    # counted once per system, as if every temperature had its own)
-    single_basis = (NT * N * 40) / launches / (launch_us * 1e-6) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                    kernel=roofline_kernel(fused), launch_us=round(launch_us, 2), alg_bytes_per_launch=alg_bytes,
-                    frac_single_lattice_basis=round(single_basis / HBM_PEAK_GBS, 4))
+                    kernel=roofline_kernel(fused), launch_us=round(launch_us, 2), alg_bytes_per_launch=alg_bytes)
     pt.close()
+    single = None if args.no_single else single_lattice(sa)
     batched = None if args.no_batch else batched_vs_plain(sa)
     served = None if args.no_serve else serving(sa, args.serve_requests, args.serve_rate)
     cpu = None if args.no_cpu else cpu_baseline()
@@ -287,6 +313,7 @@ def main():
                    "sweeps_timed": args.steps * SWEEPS},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "single_lattice": single,
         "batched_vs_plain": batched,
         "serving": served,
     }

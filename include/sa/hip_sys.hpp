@@ -112,49 +112,16 @@ class hip_sys : public gpu_sys {
   void gpu_free() override {
     if (lad_ && lad_.use_count() == 1) lad_->release();
   }
-  // reference J file: int d; long N; Jx[2N]; Jy[2N]; [Jz[2N]] (src/sys_file.cpp:10-110)
+  // reference J file: int d; long N; Jx[2N]; Jy[2N]; [Jz[2N]] (src/sys_file.cpp:10-110).
+  // As the reference, an error is logged and the couplings stay as they were
+  // (src/cpu_3d.cpp:379-399).
   void load_J(std::string fname) override {
-    std::ifstream f(fname, std::ios::in | std::ios::binary);
-    int err = 0;
-    if (!f.is_open()) err = -1;
-    int d = 0;
-    long n = 0;
-    if (!err) {
-      f.read(reinterpret_cast<char *>(&d), sizeof d);
-      if (d != dim()) err = -2;
-    }
-    if (!err) {
-      f.read(reinterpret_cast<char *>(&n), sizeof n);
-      if (n != get_N()) err = -3;
-    }
-    std::vector<uint64_t> dup[3];
-    for (int a = 0; a < dim() && !err; ++a) {
-      dup[a].resize(2 * (size_t)n);
-      f.read(reinterpret_cast<char *>(dup[a].data()), 16 * n);
-      if (!f) err = -4;
-    }
-    if (err) {
-      std::cerr << "Error:   failed to load J from \"" << fname << "\" (" << err << ")\n";
-      return;
-    }
-    std::vector<uint64_t> j[3];
-    for (int a = 0; a < dim(); ++a) {
-      j[a].resize(n);
-      for (long i = 0; i < n; ++i) j[a][i] = dup[a][2 * i];
-    }
-    check("load_J", sa_ladder_set_couplings(l(), j[0].data(), j[1].data(), dim() == 3 ? j[2].data() : nullptr));
+    if (sa_ladder_load_J(l(), fname.c_str()) != SA_OK)
+      std::cerr << "Error:   failed to load J from \"" << fname << "\" (" << sa_last_error() << ")\n";
   }
   void save_J(std::string fname) override {
-    std::ofstream f(fname, std::ios::out | std::ios::binary);
-    if (!f.is_open()) {
-      std::cerr << "Error:   failed to save J to \"" << fname << "\" (-1)\n";
-      return;
-    }
-    int d = dim();
-    long n = get_N();
-    f.write(reinterpret_cast<const char *>(&d), sizeof d);
-    f.write(reinterpret_cast<const char *>(&n), sizeof n);
-    for (auto &a : duplicated_couplings()) f.write(reinterpret_cast<const char *>(a.data()), a.size() * 8);
+    if (sa_ladder_save_J(l(), fname.c_str()) != SA_OK)
+      std::cerr << "Error:   failed to save J to \"" << fname << "\" (" << sa_last_error() << ")\n";
   }
   void init_J() override { check("init_J", sa_ladder_init_couplings(l())); }
   void init_rand() override { check("init_rand", sa_ladder_init_spins(l())); }

@@ -95,6 +95,14 @@ int sa_ladder_init_spins(sa_ladder *l);
  * (src/gpu_3d.cu:212-249) move the same bonds in the duplicated file layout. */
 int sa_ladder_set_couplings(sa_ladder *l, const uint64_t *jx, const uint64_t *jy, const uint64_t *jz);
 int sa_ladder_get_couplings(const sa_ladder *l, uint64_t *jx, uint64_t *jy, uint64_t *jz);
+/* The reference's J file (src/sys_file.cpp:10-110: int d; long N; Jx[2N]; Jy[2N];
+ * [Jz[2N]], every bond stored twice, [2i] toward -axis and [2i+1] toward +axis):
+ * load_J / save_J of gpu_3d, gpu_2d, cpu_3d, cpu_2dp (src/gpu_3d.cu:212-249,
+ * src/cpu_3d.cpp:379-399, src/cpu_2dp.cpp:268-288).  Loading checks the dimension
+ * and N (the reference's errors -2, -3) and that both copies of every bond agree; on
+ * any error the couplings stay unchanged and SA_ERR_ARG is returned. */
+int sa_ladder_load_J(sa_ladder *l, const char *path);
+int sa_ladder_save_J(const sa_ladder *l, const char *path);
 /* spins of system `sys` (N words) */
 int sa_ladder_set_spins(sa_ladder *l, int sys, const uint64_t *s);
 int sa_ladder_get_spins(const sa_ladder *l, int sys, uint64_t *s);
@@ -150,7 +158,8 @@ int sa_ladder_swap(sa_ladder *l, int a, int b, uint64_t mask);
  *                           divide nsys): more share coupling loads and random words
  *   SA_TUNE_SWEEP3D_KERNEL  1 = two half-sweep launches, 2 = fused single pass
  *   SA_TUNE_ROWS_PER_TILE   rows of the band a fused workgroup owns (4, 8, 16)
- *   SA_TUNE_XCD_MAP         1 = keep a system's bands on one XCD (-1 automatic) */
+ *   SA_TUNE_XCD_MAP         fused sweep: 1 = adjacent bands of every system on one
+ *                           XCD, 2 = whole systems on one XCD, 0 = neither (-1 automatic) */
 enum sa_tune_key {
   SA_TUNE_SYS_PER_THREAD = 1,
   SA_TUNE_SWEEP3D_KERNEL = 2,

@@ -268,6 +268,8 @@ def ref():
         R.ref_mt32.argtypes = [C.c_uint32, _u32p, C.c_long]
         R.ref_uniform_real64.argtypes = [C.c_uint64, _f64p, C.c_long]
         R.ref_pt_draws.argtypes = [C.c_int, _i32p, _f64p, C.c_long]
+        R.ref_save_J.argtypes = [C.c_void_p, C.c_char_p]
+        R.ref_load_J.argtypes = [C.c_void_p, C.c_char_p]
         R.ref_pt_run.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, C.c_double, C.c_long, C.c_int, C.c_int,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         R.ref_pt_checkpoint.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, C.c_double, C.c_long, C.c_int, C.c_int,

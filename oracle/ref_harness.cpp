@@ -101,6 +101,9 @@ uint64_t ref_peek_word(void *p, int kind) {
   std::mt19937_64 g = static_cast<cpu_2dp *>(p)->gen;
   return g();
 }
+// the system's J file (cpu_3d::save_J / cpu_2dp::save_J, src/sys_file.cpp)
+void ref_save_J(void *p, const char *path) { static_cast<cpu_sys *>(p)->save_J(path); }
+void ref_load_J(void *p, const char *path) { static_cast<cpu_sys *>(p)->load_J(path); }
 void ref_swap(void *a, void *b, uint64_t mask) {
   std::unique_ptr<spin_t[]> m(new spin_t[1]);
   m[0] = mask;

@@ -29,6 +29,7 @@ EXPORTS = [
     "sa_ladder_set_sweep_index",
     "sa_ladder_get_engine", "sa_ladder_set_engine",
     "sa_ladder_init_couplings", "sa_ladder_init_spins", "sa_ladder_set_couplings", "sa_ladder_get_couplings",
+    "sa_ladder_load_J", "sa_ladder_save_J",
     "sa_ladder_set_spins", "sa_ladder_get_spins", "sa_ladder_set_temperature", "sa_ladder_sweep",
     "sa_ladder_anneal",
     "sa_ladder_measure", "sa_ladder_measure_systems", "sa_measure_noalloc", "sa_ladder_swap", "sa_ladder_sync", "sa_ladder_stream",
@@ -112,6 +113,8 @@ def lib():
         L.sa_ladder_init_spins.argtypes = [_vp]
         L.sa_ladder_set_couplings.argtypes = [_vp, _u64p, _u64p, _vp]
         L.sa_ladder_get_couplings.argtypes = [_vp, _u64p, _u64p, _vp]
+        L.sa_ladder_load_J.argtypes = [_vp, C.c_char_p]
+        L.sa_ladder_save_J.argtypes = [_vp, C.c_char_p]
         L.sa_ladder_set_spins.argtypes = [_vp, C.c_int, _u64p]
         L.sa_ladder_get_spins.argtypes = [_vp, C.c_int, _u64p]
         L.sa_ladder_set_temperature.argtypes = [_vp, C.c_int, C.c_double, C.c_double, _vp]
@@ -284,6 +287,12 @@ class Ladder:
         jz = np.zeros(self.N, dtype=np.uint64)
         check(lib().sa_ladder_get_couplings(self.h, jx, jy, _ptr(jz)), "get_couplings")
         return jx, jy, jz
+
+    def load_J(self, path: str):
+        check(lib().sa_ladder_load_J(self.h, os.fsencode(path)), "load_J")
+
+    def save_J(self, path: str):
+        check(lib().sa_ladder_save_J(self.h, os.fsencode(path)), "save_J")
 
     def set_spins(self, sys: int, s):
         check(lib().sa_ladder_set_spins(self.h, sys, np.ascontiguousarray(s, dtype=np.uint64)), "set_spins")

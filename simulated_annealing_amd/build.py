@@ -17,6 +17,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libsa_hip.so")
+CLI = os.path.join(PKG, "sa_main")  # command-line driver with the reference's flags
 
 ARCH = os.environ.get("SA_OFFLOAD_ARCH", "gfx950")
 
@@ -45,6 +46,13 @@ def build(verbose: bool = True) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    # the command-line driver: host C++ over the library
+    cli = [hipcc(), "-O2", "-std=c++17", "-Wall", "-Wno-unknown-pragmas", "-I", INCLUDE,
+           os.path.join(PKG, "cli", "sa_main.cpp"), "-L", PKG, "-lsa_hip", "-Wl,-rpath,$ORIGIN", "-o", CLI + ".tmp"]
+    if verbose:
+        print(" ".join(cli), flush=True)
+    subprocess.check_call(cli)
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 

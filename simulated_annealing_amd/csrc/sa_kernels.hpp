@@ -187,6 +187,9 @@ __device__ __forceinline__ uint64_t bld(buf_t b, uint32_t voff, uint32_t soff) {
 // (read by two bands) keep the default policy.  Measured on the 3D L=128, 32-system
 // sweep: 309 -> 283 us per launch.
 constexpr int kCpolNT = 2;
+#ifndef SA_FUSED_PAIR_CPOL
+#define SA_FUSED_PAIR_CPOL kCpolNT
+#endif
 template <int CPOL = 0>
 __device__ __forceinline__ void bld2(buf_t b, uint32_t voff, uint32_t soff, uint64_t &a, uint64_t &c) {
   const sa_v4u32 v = __builtin_amdgcn_raw_buffer_load_b128(b, (int)voff, (int)soff, CPOL);
@@ -391,7 +394,7 @@ struct fused_ctx {
     const uint32_t vn = top ? kOutOfRange : vpair;
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
-      bld2<kCpolNT>(sin[m], vn, Pd, q.na[m], q.nb[m]);
+      bld2<SA_FUSED_PAIR_CPOL>(sin[m], vn, Pd, q.na[m], q.nb[m]);
       q.ring2[m] = bld(sin[m], vring, P);
     }
     bld2(bjx, vpair, P, q.jxa, q.jxb);
@@ -599,7 +602,7 @@ struct fused_ctx {
           for (int m = 0; m < MB; ++m) oa[m] = ob[m] = 0;
         }
 #pragma unroll
-        for (int m = 0; m < MB; ++m) bst2<kCpolNT>(sout[m], vstore, (uint32_t)z * plane_bytes, oa[m], ob[m]);
+        for (int m = 0; m < MB; ++m) bst2<SA_FUSED_PAIR_CPOL>(sout[m], vstore, (uint32_t)z * plane_bytes, oa[m], ob[m]);
       }
       if (z + 1 < L) {
         uint64_t g0, g1;
@@ -706,9 +709,16 @@ __global__ void __launch_bounds__(640) k_sweep3d_fused(sweep_args p, uint64_t *_
   // multiple of 8 every system block of a band runs on one XCD and shares its
   // couplings in L2.  xcd_map = 1: XCD b%8 gets ntile/8 adjacent bands of every
   // system block, so the halo rows between those bands are L2 hits too.
+  // xcd_map = 2: XCD b%8 gets every band of nsb/8 whole system blocks, so every halo
+  // row a band reads is read by its neighbour band on the same XCD (the couplings
+  // are then read once per XCD).
   const int b = blockIdx.x;
   int sb, tile;
-  if (xcd_map) {
+  if (xcd_map == 2) {
+    const int per = (p.nsys / MB) >> 3, xcd = b & 7, q = b >> 3;
+    sb = xcd * per + q / ntile;
+    tile = q % ntile;
+  } else if (xcd_map) {
     const int per = ntile >> 3, xcd = b & 7, q = b >> 3;
     tile = xcd * per + q % per;
     sb = q / per;

@@ -131,8 +131,12 @@ bool fused_ok(const sa_ladder *l, int th, int mb) {
   return fused_lds_bytes(l->L, th, mb) <= 64 * 1024;
 }
 
-// automatic choice: the largest band that fits (less halo recomputation), two
-// systems per workgroup when the ladder has enough of them to fill the GPU
+// automatic choice: the largest band that fits (less halo recomputation) among those
+// that give at least one workgroup per CU, two systems per workgroup when the ladder
+// has enough of them; a ladder too small to fill the GPU takes the smallest band
+// (most workgroups).  Measured (us per sweep): one L=64 system 46.7 with 4-row
+// bands, 61.0 with 8, 88.3 with 16; one L=128 system 107 / 159; 32 L=128 systems
+// 283 with 8-row bands and two systems per workgroup.
 fused_cfg pick_fused(const sa_ladder *l) {
   fused_cfg c{0, 1, 0};
   // general tables (h != 0) need more registers: one system per thread unless asked
@@ -140,10 +144,16 @@ fused_cfg pick_fused(const sa_ladder *l) {
   for (char f : l->fast) all_fast = all_fast && f;
   const int mbs[2] = {l->sys_per_thread > 0 ? l->sys_per_thread : (all_fast ? 2 : 1), 1};
   const int ths[3] = {16, 8, 4};
+  bool fill = false;  // some configuration reaches one workgroup per CU
+  for (int mb : mbs)
+    for (int th : ths)
+      if ((l->rows_per_tile <= 0 || th == l->rows_per_tile) && fused_ok(l, th, mb) &&
+          (l->L / th) * (l->nsys / mb) >= 256)
+        fill = true;
   for (int mb : mbs) {
     if (mb != 1 && mb != 2) continue;
     for (int th : ths) {
-      if (l->rows_per_tile > 0 && th != l->rows_per_tile) continue;
+      if (l->rows_per_tile > 0 ? th != l->rows_per_tile : (!fill && th != 4)) continue;
       if (!fused_ok(l, th, mb)) continue;
       // two systems per workgroup share couplings and random words, unless that
       // leaves fewer workgroups than CUs
@@ -151,8 +161,9 @@ fused_cfg pick_fused(const sa_ladder *l) {
       c.th = th;
       c.mb = mb;
       const int ntile = l->L / th;
-      const bool can = ntile % 8 == 0;
-      c.xcd = (l->xcd_map < 0) ? can : (l->xcd_map && can);  // measured faster where it applies
+      const bool can = ntile % 8 == 0, can2 = (l->nsys / mb) % 8 == 0;
+      if (l->xcd_map == 2) c.xcd = can2 ? 2 : 0;
+      else c.xcd = (l->xcd_map < 0) ? can : (l->xcd_map && can);  // measured faster where it applies
       return c;
     }
   }
@@ -383,6 +394,75 @@ int sa_ladder_get_couplings(const sa_ladder *l, uint64_t *jx, uint64_t *jy, uint
   if (l->kind == SA_KIND_3D) SA_HIP_TRY(hipMemcpyAsync(jz, l->jz, bytes, hipMemcpyDeviceToHost, l->stream));
   SA_HIP_TRY(hipStreamSynchronize(l->stream));
   return SA_OK;
+}
+
+// J file of the reference (src/sys_file.cpp:10-110): int d; long N; then per axis 2N
+// words, [2i] = the bond of site i toward -axis, [2i+1] = toward +axis (the +axis
+// neighbour's -axis bond, periodic wrap; src/cpu_3d.cpp:290-299).
+namespace {
+int64_t neighbour_plus(int64_t i, int axis, int L, int64_t N) {
+  const int64_t x = i % L, y = (i / L) % L, L2 = (int64_t)L * L;
+  if (axis == 0) return i - x + (x + 1) % L;
+  if (axis == 1) return i - y * L + ((y + 1) % L) * L;
+  return (i + L2) % N;
+}
+}  // namespace
+
+int sa_ladder_load_J(sa_ladder *l, const char *path) {
+  if (!l || !path) return sa_fail(SA_ERR_ARG, "null argument");
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return sa_fail(SA_ERR_ARG, "load_J: cannot open \"%s\"", path);
+  int d = 0;
+  long n = 0;
+  const int dim = l->kind == SA_KIND_3D ? 3 : 2;
+  int rc = SA_OK;
+  if (std::fread(&d, sizeof d, 1, f) != 1 || std::fread(&n, sizeof n, 1, f) != 1)
+    rc = sa_fail(SA_ERR_ARG, "load_J: \"%s\" is shorter than its header", path);
+  else if (d != dim)
+    rc = sa_fail(SA_ERR_ARG, "load_J: \"%s\" holds d=%d couplings, the ladder is %dD", path, d, dim);
+  else if (n != l->N)
+    rc = sa_fail(SA_ERR_ARG, "load_J: \"%s\" holds N=%ld sites, the ladder has %lld", path, n, (long long)l->N);
+  std::vector<uint64_t> dup((size_t)2 * l->N), j[3];
+  for (int a = 0; a < dim && rc == SA_OK; ++a) {
+    if (std::fread(dup.data(), 8, dup.size(), f) != dup.size()) {
+      rc = sa_fail(SA_ERR_ARG, "load_J: \"%s\" ends inside the couplings", path);
+      break;
+    }
+    j[a].resize(l->N);
+    for (int64_t i = 0; i < l->N; ++i) j[a][i] = dup[2 * i];
+    // each bond is stored twice; a file whose copies disagree has no deduplicated form
+    for (int64_t i = 0; i < l->N && rc == SA_OK; ++i)
+      if (dup[2 * i + 1] != dup[2 * neighbour_plus(i, a, l->L, l->N)])
+        rc = sa_fail(SA_ERR_ARG, "load_J: \"%s\": axis %d, site %lld: the two copies of a bond differ", path, a,
+                     (long long)i);
+  }
+  std::fclose(f);
+  if (rc != SA_OK) return rc;  // couplings unchanged
+  return sa_ladder_set_couplings(l, j[0].data(), j[1].data(), dim == 3 ? j[2].data() : nullptr);
+}
+
+int sa_ladder_save_J(const sa_ladder *l, const char *path) {
+  if (!l || !path) return sa_fail(SA_ERR_ARG, "null argument");
+  const int dim = l->kind == SA_KIND_3D ? 3 : 2;
+  std::vector<uint64_t> j[3];
+  for (auto &a : j) a.resize(l->N);
+  int rc = sa_ladder_get_couplings(l, j[0].data(), j[1].data(), j[2].data());
+  if (rc != SA_OK) return rc;
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return sa_fail(SA_ERR_ARG, "save_J: cannot open \"%s\"", path);
+  const int d = dim;
+  const long n = (long)l->N;
+  bool ok = std::fwrite(&d, sizeof d, 1, f) == 1 && std::fwrite(&n, sizeof n, 1, f) == 1;
+  std::vector<uint64_t> dup((size_t)2 * l->N);
+  for (int a = 0; a < dim && ok; ++a) {
+    for (int64_t i = 0; i < l->N; ++i) {
+      dup[2 * i] = j[a][i];
+      dup[2 * i + 1] = j[a][neighbour_plus(i, a, l->L, l->N)];
+    }
+    ok = std::fwrite(dup.data(), 8, dup.size(), f) == dup.size();
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? SA_OK : sa_fail(SA_ERR_ARG, "save_J: writing \"%s\" failed", path);
 }
 
 int sa_ladder_init_couplings(sa_ladder *l) {
@@ -656,7 +736,7 @@ int sa_ladder_set_tuning(sa_ladder *l, int key, int value) {
       l->rows_per_tile = value;
       return SA_OK;
     case SA_TUNE_XCD_MAP:
-      if (value < -1 || value > 1) return sa_fail(SA_ERR_ARG, "XCD map must be -1 (auto), 0 or 1");
+      if (value < -1 || value > 2) return sa_fail(SA_ERR_ARG, "XCD map must be -1 (auto), 0, 1 or 2");
       l->xcd_map = value;
       return SA_OK;
     default:

@@ -7,6 +7,7 @@ sources).  Run in the container that has the reference sources:
     python tests/golden/make_golden.py --pt-files      # pt_files.json
     python tests/golden/make_golden.py --ground        # pt_ground.json
     python tests/golden/make_golden.py --configs       # config_vectors.json
+    python tests/golden/make_golden.py --jfiles        # jfile_k*_L*.J (the reference's J files)
 
 The fixtures are data only: inputs, seeds and the reference's outputs.  Spin arrays
 are stored whole (hex words) for small lattices and as SHA-256 of their
@@ -243,9 +244,29 @@ def config_runs():
     return out
 
 
+JFILES = [(2, 6, 1234), (1, 8, 77), (0, 10, 5)]  # (kind, L, seed) of the J-file fixtures
+
+
+def jfiles():
+    """J files written by the reference's own save_J (src/sys_file.cpp:10-110) for a
+    system seeded like main (set_seed, init_J): jfile_k<kind>_L<L>.J."""
+    R = O.ref()
+    for kind, L, seed in JFILES:
+        p = R.ref_create(kind, L, 1.0, 0.0)
+        R.ref_set_seed(p, seed)
+        R.ref_init_J(p)
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"jfile_k{kind}_L{L}.J")
+        R.ref_save_J(p, path.encode())
+        R.ref_destroy(p)
+        print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def main():
     if not O.ref_available():
         O.build_ref()
+    if "--jfiles" in sys.argv:
+        jfiles()
+        return
     if "--configs" in sys.argv:  # only the benchmark-configuration fixtures
         with open(CONFIG_OUT, "w") as f:
             json.dump(dict(about="The benchmark configurations run by the reference (see make_golden.py).",
