@@ -156,7 +156,9 @@ int sa_ladder_swap(sa_ladder *l, int a, int b, uint64_t mask);
 /* Kernel tuning (no effect on results; 0 = automatic everywhere).
  *   SA_TUNE_SYS_PER_THREAD  systems one thread updates per site (1, 2 or 4, must
  *                           divide nsys): more share coupling loads and random words
- *   SA_TUNE_SWEEP3D_KERNEL  1 = two half-sweep launches, 2 = fused single pass
+ *   SA_TUNE_SWEEP3D_KERNEL  1 = two half-sweep launches, 2 = fused single pass,
+ *                           3 = two z-parallel half-sweep launches (even L; column
+ *                           walks split into segments joined by a scan)
  *   SA_TUNE_ROWS_PER_TILE   rows of the band a fused workgroup owns (4, 8, 16)
  *   SA_TUNE_XCD_MAP         fused sweep: 1 = adjacent bands of every system on one
  *                           XCD, 2 = whole systems on one XCD, 0 = neither (-1 automatic) */

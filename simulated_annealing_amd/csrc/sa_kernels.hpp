@@ -729,6 +729,144 @@ __global__ void __launch_bounds__(640) k_sweep3d_fused(sweep_args p, uint64_t *_
   fused_band<COUNTER, FAST, MB, TH, WALK>(p, s_out, sb, tile, lds);
 }
 
+// ---------------------------------------------------------------- 3D half-sweep, z in parallel
+// For a lattice too small to fill the GPU with column walks (one system offers L*L/2
+// columns per half-sweep), each column is cut into G segments of S = L/G planes, one
+// thread per segment, the G threads of a column in one wave: a wave holds C = 64/G
+// consecutive columns, lane = segment*C + column, so C neighbouring lanes read
+// neighbouring columns of one plane.
+// In a half-sweep the in-plane neighbours have the other colour and stay fixed, and
+// the old +z word is fixed, so the new word of plane z is a per-lane function of the
+// new word of plane z-1 only: one of the four maps of a bit, kept as the pair
+// (f0, f1) = (result for a 0 below, result for a 1 below) -- two evaluations of the
+// update with the -z neighbour all zeros and all ones, sharing the site's random
+// word.  Plane 0 reads the old plane L-1, so its map is constant; plane L-1 reads the
+// new plane 0 as its +z neighbour, which the segment-0 lane has computed first and
+// passes on.  Maps compose as (F o G)(b) = G(b) ? F1 : F0 per bit; a scan over the
+// G lanes gives every segment the new word below it, and each segment then applies
+// its maps in order and stores.  The result is the reference's in-order z walk bit
+// for bit (src/cpu_3d.cpp:94-105), with the same random word per site.  All loads of
+// a wave complete before its scan, so no segment's store meets a read of another.
+struct zmap {
+  uint64_t f0, f1;
+};
+__device__ __forceinline__ zmap zcompose(const zmap &F, const zmap &G) {  // F o G
+  return zmap{sel(G.f0, F.f1, F.f0), sel(G.f1, F.f1, F.f0)};
+}
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d, int width) {
+  const uint32_t lo = __shfl_up((uint32_t)v, d, width), hi = __shfl_up((uint32_t)(v >> 32), d, width);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+  const uint32_t lo = __shfl((uint32_t)v, src), hi = __shfl((uint32_t)(v >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// segments per column for an even L: a power of two G <= 64 with L/G <= kZsegMax
+// planes per segment and at least two planes; 0 when there is none
+constexpr int kZsegMax = 8;
+__host__ __device__ __forceinline__ int zscan_groups(int L) {
+  for (int G = 2; G <= 64; G *= 2)
+    if (L % G == 0 && L / G <= 4 && L / G >= 2) return G;
+  for (int G = 2; G <= 64; G *= 2)
+    if (L % G == 0 && L / G <= kZsegMax && L / G >= 2) return G;
+  return 0;
+}
+
+template <bool COUNTER, bool FAST, int S>
+__device__ __forceinline__ void zscan_segment(const sweep_args &p, int color, int k, int g, int G, int sys, bool live) {
+  const int L = p.L, half = L >> 1;
+  const int y = k / half;
+  const int x = 2 * (k - y * half) + ((y + color) & 1);
+  const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
+  const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
+  const int64_t L2 = (int64_t)L * L, N = p.N;
+  const int c0 = x + L * y, cw = xm + L * y, ce = xp + L * y, cn = x + L * ym, cs = x + L * yp;
+  uint64_t *s = p.s + (int64_t)sys * N;
+  const sys_table &t = p.tab[sys];
+  const int z0 = g * S;
+  const int lane = threadIdx.x & 63, C = 64 / G, base = lane % C;  // base: the column's segment-0 lane
+  auto eval = [&](uint64_t g64, const uint64_t *q, uint64_t si, uint64_t bd, uint64_t bu) {
+    const count3 cc = count6(q[0], q[1], q[2], q[3], bd, bu);
+    return FAST ? update_fast3(t, g64, cc, si) : update_general(t, 0, g64, cc, si, 7);
+  };
+  zmap F[S];
+  // plane L-1 (last plane of the last segment) waits for the new plane 0: its
+  // in-plane bonds, -z bond word with a 0 below, +z coupling and random word
+  uint64_t lq[4] = {0, 0, 0, 0}, lsi = 0, lbu0 = 0, ljd = 0, lg = 0;
+  u32x4 blk{0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const int z = z0 + j;
+    const int64_t P = z * L2;
+    uint64_t f0 = 0, f1 = 0;
+    if (live) {
+      const uint64_t si = s[P + c0];
+      const uint64_t w = s[P + cw], e = s[P + ce], n = s[P + cn], so = s[P + cs];
+      const uint64_t jw = p.jx[P + c0], je = p.jx[P + ce], jn = p.jy[P + c0], js = p.jy[P + cs];
+      const uint64_t ju = p.jz[P + c0];
+      const uint64_t jd = p.jz[(z == L - 1 ? 0 : P + L2) + c0];
+      uint64_t g64;
+      if (COUNTER) {
+        if (j == 0 || (z & 1) == 0) blk = counter_block((uint32_t)(c0 + (z & ~1) * L2), p.seed, p.sweep);
+        g64 = counter_half(blk, z & 1);
+      } else {
+        g64 = p.words[(int64_t)color * (N >> 1) + k + z * (L2 >> 1)];
+      }
+      const uint64_t q[4] = {xor3(jw, si, w), xor3(je, si, e), xor3(jn, si, n), xor3(js, si, so)};
+      const uint64_t bu0 = ju ^ si;  // -z bond word with a 0 below
+      if (z == 0) {  // -z neighbour: the old plane L-1 -- a constant map
+        const uint64_t up = s[(L - 1) * L2 + c0];
+        f0 = f1 = eval(g64, q, si, xor3(jd, si, s[P + L2 + c0]), xor3(ju, si, up));
+      } else if (z < L - 1) {  // +z neighbour: the old plane z+1
+        const uint64_t bd = xor3(jd, si, s[P + L2 + c0]);
+        f0 = eval(g64, q, si, bd, bu0);
+        f1 = eval(g64, q, si, bd, ~bu0);
+      } else {
+        for (int r = 0; r < 4; ++r) lq[r] = q[r];
+        lsi = si;
+        lbu0 = bu0;
+        ljd = jd;
+        lg = g64;
+      }
+    }
+    F[j] = zmap{f0, f1};
+  }
+  // the new plane 0 (segment 0's constant map) to every lane of the column
+  const uint64_t new0 = shfl64(F[0].f0, base);
+  if (live && z0 + S == L) {
+    const uint64_t bd = xor3(ljd, lsi, new0);
+    F[S - 1] = zmap{eval(lg, lq, lsi, bd, lbu0), eval(lg, lq, lsi, bd, ~lbu0)};
+  }
+  // the segment's map: F[S-1] o ... o F[0]
+  zmap M = F[0];
+#pragma unroll
+  for (int j = 1; j < S; ++j) M = zcompose(F[j], M);
+  // inclusive scan over the column's G lanes (C apart): segment g ends with
+  // G_g o ... o G_0, a constant map (G_0 is one), i.e. the new word of its last plane
+  for (int d = 1; d < G; d *= 2) {
+    const zmap T{shfl_up64(M.f0, d * C, 64), shfl_up64(M.f1, d * C, 64)};
+    if (g >= d) M = zcompose(M, T);
+  }
+  uint64_t v = shfl_up64(M.f0, C, 64);  // the new word below the segment's first plane
+  if (!live) return;
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    v = (g == 0 && j == 0) ? F[0].f0 : sel(v, F[j].f1, F[j].f0);
+    s[(int64_t)(z0 + j) * L2 + c0] = v;
+  }
+}
+
+// one launch per colour; thread = (column, segment), grid.y = system
+template <bool COUNTER, bool FAST, int S>
+__global__ void __launch_bounds__(256) k_sweep3d_zscan(sweep_args p, int color, int G) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int C = 64 / G, lane = i & 63;
+  const int k = (i >> 6) * C + lane % C, g = lane / C;
+  const bool live = k < p.L * (p.L >> 1);
+  zscan_segment<COUNTER, FAST, S>(p, color, live ? k : 0, g, G, blockIdx.y, live);
+}
+
 // ---------------------------------------------------------------- 2D half-sweep
 // site k (0 <= k < L*L/2) of colour `color`, systems sys0 .. sys0+MB-1
 template <bool COUNTER, bool FAST, bool OPEN, int MB>

@@ -86,8 +86,48 @@ int pick_mb(const sa_ladder *l) {
   return (l->nsys % 4 == 0) ? 4 : (l->nsys % 2 == 0) ? 2 : 1;
 }
 
+template <bool COUNTER, bool FAST, int S>
+void launch_zscan_s(const sa::sweep_args &a, int color, int G, hipStream_t st) {
+  const int64_t items = (int64_t)a.L * (a.L / 2) * G;
+  const dim3 grid((unsigned)((items + 255) / 256), (unsigned)a.nsys);
+  hipLaunchKernelGGL((sa::k_sweep3d_zscan<COUNTER, FAST, S>), grid, dim3(256), 0, st, a, color, G);
+}
+
+// z-parallel half-sweep (3D, even L with zscan_groups(L) > 0)
+template <bool COUNTER, bool FAST>
+void launch_zscan(const sa_ladder *l, const sa::sweep_args &a, int color) {
+  const int G = sa::zscan_groups(l->L), S = l->L / G;
+  switch (S) {
+    case 2: launch_zscan_s<COUNTER, FAST, 2>(a, color, G, l->stream); break;
+    case 3: launch_zscan_s<COUNTER, FAST, 3>(a, color, G, l->stream); break;
+    case 4: launch_zscan_s<COUNTER, FAST, 4>(a, color, G, l->stream); break;
+    case 5: launch_zscan_s<COUNTER, FAST, 5>(a, color, G, l->stream); break;
+    case 6: launch_zscan_s<COUNTER, FAST, 6>(a, color, G, l->stream); break;
+    case 7: launch_zscan_s<COUNTER, FAST, 7>(a, color, G, l->stream); break;
+    default: launch_zscan_s<COUNTER, FAST, 8>(a, color, G, l->stream); break;
+  }
+}
+
+bool zscan_ok(const sa_ladder *l) {
+  return l->kind == SA_KIND_3D && !(l->L & 1) && sa::zscan_groups(l->L) > 0 && l->N < ((int64_t)1 << 31);
+}
+
+// the z-parallel half-sweeps: asked for, or automatically for ladders of at most 2^21
+// site words in all, where they beat the fused walk (us per sweep, fused vs z-parallel:
+// one L=64 system 46.7 / 23.1, four 47.1 / 41.6; one L=128 system 107 / 70; two L=128
+// systems 108 / 134)
+bool use_zscan(const sa_ladder *l) {
+  if (!zscan_ok(l)) return false;
+  if (l->sweep3d_kernel == 3) return true;
+  return l->sweep3d_kernel == 0 && l->N * l->nsys <= ((int64_t)1 << 21);
+}
+
 template <bool COUNTER, bool FAST>
 void launch_half(const sa_ladder *l, const sa::sweep_args &a, int color) {
+  if (use_zscan(l)) {
+    launch_zscan<COUNTER, FAST>(l, a, color);
+    return;
+  }
   if (l->L & 1) {  // odd L: 3D one lane per member, 2D one thread per wrap group; one system per grid row
     const int items = l->kind == SA_KIND_3D ? sa::odd_width(l->L) : sa::odd_groups(l->L);
     const dim3 grid((unsigned)((items + 255) / 256), (unsigned)l->nsys);
@@ -171,7 +211,7 @@ fused_cfg pick_fused(const sa_ladder *l) {
 }
 
 bool use_fused(const sa_ladder *l) {
-  if (l->kind != SA_KIND_3D || l->sweep3d_kernel == 1) return false;
+  if (l->kind != SA_KIND_3D || l->sweep3d_kernel == 1 || use_zscan(l)) return false;
   return pick_fused(l).th > 0;
 }
 
@@ -727,7 +767,7 @@ int sa_ladder_set_tuning(sa_ladder *l, int key, int value) {
       l->sys_per_thread = value;
       return SA_OK;
     case SA_TUNE_SWEEP3D_KERNEL:
-      if (value < 0 || value > 2) return sa_fail(SA_ERR_ARG, "3D sweep kernel must be 0 (auto), 1 or 2");
+      if (value < 0 || value > 3) return sa_fail(SA_ERR_ARG, "3D sweep kernel must be 0 (auto), 1, 2 or 3");
       l->sweep3d_kernel = value;
       return SA_OK;
     case SA_TUNE_ROWS_PER_TILE:
@@ -750,7 +790,9 @@ int sa_ladder_get_tuning(const sa_ladder *l, int key, int *value) {
   const fused_cfg c = fused ? pick_fused(l) : fused_cfg{0, 0, 0};
   switch (key) {
     case SA_TUNE_SYS_PER_THREAD: *value = fused ? c.mb : pick_mb(l); return SA_OK;
-    case SA_TUNE_SWEEP3D_KERNEL: *value = l->kind != SA_KIND_3D ? 0 : (fused ? 2 : 1); return SA_OK;
+    case SA_TUNE_SWEEP3D_KERNEL:
+      *value = l->kind != SA_KIND_3D ? 0 : (fused ? 2 : use_zscan(l) ? 3 : 1);
+      return SA_OK;
     case SA_TUNE_ROWS_PER_TILE: *value = c.th; return SA_OK;
     case SA_TUNE_XCD_MAP: *value = c.xcd; return SA_OK;
     default: return sa_fail(SA_ERR_ARG, "unknown tuning key %d", key);
