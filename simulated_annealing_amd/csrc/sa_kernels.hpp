@@ -790,53 +790,54 @@ __device__ __forceinline__ void zscan_segment(const sweep_args &p, int color, in
     const count3 cc = count6(q[0], q[1], q[2], q[3], bd, bu);
     return FAST ? update_fast3(t, g64, cc, si) : update_general(t, 0, g64, cc, si, 7);
   };
-  zmap F[S];
-  // plane L-1 (last plane of the last segment) waits for the new plane 0: its
-  // in-plane bonds, -z bond word with a 0 below, +z coupling and random word
-  uint64_t lq[4] = {0, 0, 0, 0}, lsi = 0, lbu0 = 0, ljd = 0, lg = 0;
-  u32x4 blk{0, 0, 0, 0};
+  // every load of the segment first (a dead lane reads column 0: valid addresses),
+  // then the arithmetic: the planes' memory round trips overlap
+  uint64_t si[S], nb[S][4], jb[S][4], ju[S], jd[S], dn[S], g64[S], q[S][4];
 #pragma unroll
   for (int j = 0; j < S; ++j) {
     const int z = z0 + j;
     const int64_t P = z * L2;
-    uint64_t f0 = 0, f1 = 0;
-    if (live) {
-      const uint64_t si = s[P + c0];
-      const uint64_t w = s[P + cw], e = s[P + ce], n = s[P + cn], so = s[P + cs];
-      const uint64_t jw = p.jx[P + c0], je = p.jx[P + ce], jn = p.jy[P + c0], js = p.jy[P + cs];
-      const uint64_t ju = p.jz[P + c0];
-      const uint64_t jd = p.jz[(z == L - 1 ? 0 : P + L2) + c0];
-      uint64_t g64;
-      if (COUNTER) {
-        if (j == 0 || (z & 1) == 0) blk = counter_block((uint32_t)(c0 + (z & ~1) * L2), p.seed, p.sweep);
-        g64 = counter_half(blk, z & 1);
-      } else {
-        g64 = p.words[(int64_t)color * (N >> 1) + k + z * (L2 >> 1)];
-      }
-      const uint64_t q[4] = {xor3(jw, si, w), xor3(je, si, e), xor3(jn, si, n), xor3(js, si, so)};
-      const uint64_t bu0 = ju ^ si;  // -z bond word with a 0 below
-      if (z == 0) {  // -z neighbour: the old plane L-1 -- a constant map
-        const uint64_t up = s[(L - 1) * L2 + c0];
-        f0 = f1 = eval(g64, q, si, xor3(jd, si, s[P + L2 + c0]), xor3(ju, si, up));
-      } else if (z < L - 1) {  // +z neighbour: the old plane z+1
-        const uint64_t bd = xor3(jd, si, s[P + L2 + c0]);
-        f0 = eval(g64, q, si, bd, bu0);
-        f1 = eval(g64, q, si, bd, ~bu0);
-      } else {
-        for (int r = 0; r < 4; ++r) lq[r] = q[r];
-        lsi = si;
-        lbu0 = bu0;
-        ljd = jd;
-        lg = g64;
-      }
+    si[j] = s[P + c0];
+    nb[j][0] = s[P + cw];
+    nb[j][1] = s[P + ce];
+    nb[j][2] = s[P + cn];
+    nb[j][3] = s[P + cs];
+    jb[j][0] = p.jx[P + c0];
+    jb[j][1] = p.jx[P + ce];
+    jb[j][2] = p.jy[P + c0];
+    jb[j][3] = p.jy[P + cs];
+    ju[j] = p.jz[P + c0];
+    jd[j] = p.jz[(z == L - 1 ? 0 : P + L2) + c0];
+    dn[j] = s[(z == L - 1 ? 0 : P + L2) + c0];  // plane L-1: replaced by the new plane 0 below
+    if (!COUNTER) g64[j] = p.words[(int64_t)color * (N >> 1) + k + z * (L2 >> 1)];
+  }
+  const uint64_t up0 = s[(L - 1) * L2 + c0];  // old plane L-1: plane 0's -z neighbour
+  if (COUNTER) {
+    u32x4 blk{0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const int z = z0 + j;
+      if (j == 0 || (z & 1) == 0) blk = counter_block((uint32_t)(c0 + (z & ~1) * L2), p.seed, p.sweep);
+      g64[j] = counter_half(blk, z & 1);
     }
-    F[j] = zmap{f0, f1};
+  }
+  zmap F[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const int z = z0 + j;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) q[j][r] = xor3(jb[j][r], si[j], nb[j][r]);
+    const uint64_t bu0 = z == 0 ? xor3(ju[j], si[j], up0) : (ju[j] ^ si[j]);  // -z bond word, 0 below
+    const uint64_t bd = xor3(jd[j], si[j], dn[j]);
+    F[j].f0 = eval(g64[j], q[j], si[j], bd, bu0);
+    F[j].f1 = z == 0 ? F[j].f0 : eval(g64[j], q[j], si[j], bd, ~bu0);  // plane 0: a constant map
   }
   // the new plane 0 (segment 0's constant map) to every lane of the column
   const uint64_t new0 = shfl64(F[0].f0, base);
-  if (live && z0 + S == L) {
-    const uint64_t bd = xor3(ljd, lsi, new0);
-    F[S - 1] = zmap{eval(lg, lq, lsi, bd, lbu0), eval(lg, lq, lsi, bd, ~lbu0)};
+  if (z0 + S == L) {  // plane L-1: its +z neighbour is the new plane 0
+    constexpr int j = S - 1;
+    const uint64_t bd = xor3(jd[j], si[j], new0), bu0 = ju[j] ^ si[j];
+    F[j] = zmap{eval(g64[j], q[j], si[j], bd, bu0), eval(g64[j], q[j], si[j], bd, ~bu0)};
   }
   // the segment's map: F[S-1] o ... o F[0]
   zmap M = F[0];
