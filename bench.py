@@ -101,7 +101,9 @@ def batched_vs_plain(sa, n=512):
     (one launch per half-sweep for all unfinished instances, hipGraph chunks, block
     size and chunk length tuned online inside the timed region).  Plain: a ladder
     per instance driven by sa_ladder_anneal (temperature upload + sweep launch per
-    sweep, each instance on its own stream).  Set-up (seeding, couplings) is outside
+    sweep, each instance on its own stream; 3D instances on the fused sweep, one
+    launch per sweep, which is faster here than the two z-parallel launches a lone
+    small ladder takes by default).  Set-up (seeding, couplings) is outside
     both timed regions.  Every instance's final spins are compared bit for bit."""
     from simulated_annealing_amd import workloads
 
@@ -117,6 +119,8 @@ def batched_vs_plain(sa, n=512):
         lad = sa.Ladder(w.kind, w.L, 1)
         lad.set_rng(sa.RNG_COUNTER)
         lad.seed_like_main(w.seed)
+        if w.kind == sa.KIND_3D:  # one launch per sweep: the plain path's fastest form here
+            lad.tune(sa.TUNE_SWEEP3D_KERNEL, 2)
         lads.append(lad)
     for lad in lads:
         lad.sync()

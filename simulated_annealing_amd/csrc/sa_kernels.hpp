@@ -515,16 +515,15 @@ struct fused_ctx {
     ju1 = bs.jd;
   }
 
-  // old pair of plane q into LDS buffer q%3 (colour-0 and colour-1 slots)
+  // old colour-1 word of plane q into LDS buffer q%3.  The old colour-0 word is not
+  // stored: nothing reads it from LDS (phase A of plane q writes the new one into the
+  // same slot before any phase B reads the slot).
   template <int PT>
   __device__ __forceinline__ void put_old(int q, const uint64_t *pa, const uint64_t *pb) const {
     const int P = PT >= 0 ? PT : par;
     const int buf = q % 3;
 #pragma unroll
-    for (int m = 0; m < MB; ++m) {
-      at(slab(buf, m, 0), t_rk) = P ? pb[m] : pa[m];
-      at(slab(buf, m, 1), t_rk) = P ? pa[m] : pb[m];
-    }
+    for (int m = 0; m < MB; ++m) at(slab(buf, m, 1), t_rk) = P ? pa[m] : pb[m];
   }
 
   // random words of the colour-0 / colour-1 site at plane q
